@@ -1,0 +1,273 @@
+"""ctypes binding of the C ABI in include/cyrsoxs.h (lib/libcyrsoxs.so).
+
+This is the Python view of the drop-in boundary: plain pointers and POD
+descriptors.  Device buffers are torch tensors (PyTorch supplies HBM
+allocations and streams; all compute is in the HIP library).  Loading fails
+loudly when the library has not been built: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libcyrsoxs.so")
+
+CYR_OK = 0
+STATUS = {0: "CYR_OK", 1: "CYR_E_INVALID", 2: "CYR_E_NOMEM", 3: "CYR_E_DEVICE", 4: "CYR_E_ADMISSION",
+          5: "CYR_E_CANCELLED", 6: "CYR_E_UNSUPPORTED", 7: "CYR_E_NODEVICE", 8: "CYR_E_TIMEOUT",
+          9: "CYR_E_NOTFOUND"}
+FP32, FP64 = 0, 1
+STORE_NATIVE, STORE_FP16, STORE_BF16 = 0, 1, 2
+PIPE_PLAIN, PIPE_EBASIS = 0, 1
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+class SimDesc(ctypes.Structure):
+    _fields_ = [
+        ("dims", ctypes.c_uint32 * 3),
+        ("num_materials", ctypes.c_uint32),
+        ("phys_size", ctypes.c_double),
+        ("num_energies", ctypes.c_uint32),
+        ("energies", _dp),
+        ("angle_start", ctypes.c_double),
+        ("angle_increment", ctypes.c_double),
+        ("angle_end", ctypes.c_double),
+        ("num_kvectors", ctypes.c_uint32),
+        ("kvectors", _dp),
+        ("detector", ctypes.c_double * 3),
+        ("case_type", ctypes.c_int32),
+        ("morphology_type", ctypes.c_int32),
+        ("reference_frame", ctypes.c_int32),
+        ("interpolation", ctypes.c_int32),
+        ("windowing", ctypes.c_int32),
+        ("rot_mask", ctypes.c_int32),
+        ("scatter_approach", ctypes.c_int32),
+        ("algorithm", ctypes.c_int32),
+        ("precision", ctypes.c_int32),
+        ("storage", ctypes.c_int32),
+        ("pipeline", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
+class EngineStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64), ("rejected", ctypes.c_uint64),
+                ("cancelled", ctypes.c_uint64), ("resident_bytes", ctypes.c_uint64),
+                ("budget_bytes", ctypes.c_uint64), ("in_flight", ctypes.c_uint32), ("queued", ctypes.c_uint32)]
+
+
+class CyrError(RuntimeError):
+    def __init__(self, status, where, detail=""):
+        self.status = status
+        super().__init__(f"{where}: {STATUS.get(status, status)} {detail}".strip())
+
+
+_lib = None
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "cyr_simulate": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "cyr_simulate_device": (ctypes.c_int, [ctypes.c_void_p] * 5),
+    "cyr_polarization": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
+                                        ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "cyr_polarization_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
+                                               ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "cyr_fft3d_device": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int32,
+                                        ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p]),
+    "cyr_dc_shift_device": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int32,
+                                           ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p]),
+    "cyr_ewald_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p,
+                                        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "cyr_warp_affine_device": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int32, ctypes.c_void_p,
+                                              ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "cyr_simulate_batch": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                          ctypes.c_void_p, ctypes.c_void_p]),
+    "cyr_engine_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_void_p]),
+    "cyr_engine_submit": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.c_void_p, ctypes.c_void_p]),
+    "cyr_engine_wait": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_double, ctypes.c_void_p]),
+    "cyr_engine_poll": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p]),
+    "cyr_engine_cancel": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64]),
+    "cyr_engine_get_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "cyr_engine_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "cyr_status_string": (ctypes.c_char_p, [ctypes.c_int]),
+    "cyr_last_error": (ctypes.c_char_p, []),
+    "cyr_device_count": (ctypes.c_int, []),
+    "cyr_num_angles": (ctypes.c_uint32, [ctypes.c_double, ctypes.c_double, ctypes.c_double]),
+    "cyr_footprint_bytes": (ctypes.c_uint64, [ctypes.c_void_p]),
+    "cyr_release_workspace": (ctypes.c_int, []),
+    "cyr_version": (ctypes.c_char_p, []),
+}
+
+
+def exported_symbols():
+    return sorted(_SIGS)
+
+
+def lib():
+    """Loads lib/libcyrsoxs.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with `python -m cyrsoxs_amd.build` "
+                              "(the HIP library is the only implementation)")
+        L = ctypes.CDLL(LIB_PATH, mode=ctypes.RTLD_GLOBAL)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(status, where):
+    if status != CYR_OK:
+        detail = lib().cyr_last_error()
+        raise CyrError(status, where, detail.decode() if detail else "")
+
+
+class Sim:
+    """Keeps a SimDesc and the arrays it points to alive together."""
+
+    def __init__(self, dims, num_materials, phys_size, energies, angles, kvectors=None, detector=(0, 0, 1),
+                 case_type=0, morphology_type=0, reference_frame=1, interpolation=1, windowing=0, rot_mask=0,
+                 scatter_approach=0, algorithm=0, precision=FP32, storage=STORE_NATIVE, pipeline=PIPE_PLAIN):
+        self.energies = np.ascontiguousarray(energies, dtype=np.float64)
+        kv = np.zeros((0, 3)) if kvectors is None else np.asarray(kvectors, dtype=np.float64).reshape(-1, 3)
+        self.kvectors = np.ascontiguousarray(kv)
+        d = SimDesc()
+        d.dims[0], d.dims[1], d.dims[2] = (int(x) for x in dims)
+        d.num_materials = int(num_materials)
+        d.phys_size = float(phys_size)
+        d.num_energies = len(self.energies)
+        d.energies = self.energies.ctypes.data_as(_dp)
+        d.angle_start, d.angle_increment, d.angle_end = (float(x) for x in angles)
+        d.num_kvectors = len(self.kvectors)
+        d.kvectors = self.kvectors.ctypes.data_as(_dp) if len(self.kvectors) else None
+        for i in range(3):
+            d.detector[i] = float(detector[i])
+        d.case_type = case_type
+        d.morphology_type = morphology_type
+        d.reference_frame = reference_frame
+        d.interpolation = interpolation
+        d.windowing = windowing
+        d.rot_mask = rot_mask
+        d.scatter_approach = scatter_approach
+        d.algorithm = algorithm
+        d.precision = precision
+        d.storage = storage
+        d.pipeline = pipeline
+        self.desc = d
+
+    @property
+    def ref(self):
+        return ctypes.byref(self.desc)
+
+    @property
+    def dims(self):
+        return tuple(self.desc.dims)
+
+    @property
+    def num_angles(self):
+        return int(lib().cyr_num_angles(self.desc.angle_start, self.desc.angle_increment, self.desc.angle_end))
+
+    @property
+    def num_k(self):
+        return max(1, len(self.kvectors))
+
+    @property
+    def real_dtype(self):
+        return np.float64 if self.desc.precision == FP64 else np.float32
+
+    def footprint(self):
+        return int(lib().cyr_footprint_bytes(self.ref))
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def simulate(sim: Sim, voxels, mats):
+    """Host API: voxels [M][V][4] (compute precision), mats [nE][M][4] -> patterns [nE][nK][Ny][Nx]."""
+    rd = sim.real_dtype
+    vox = np.ascontiguousarray(voxels, dtype=rd)
+    m = np.ascontiguousarray(mats, dtype=np.float64)
+    nx, ny, nz = sim.dims
+    out = np.empty((len(sim.energies), sim.num_k, ny, nx), dtype=rd)
+    check(lib().cyr_simulate(sim.ref, _ptr(vox), _ptr(m), _ptr(out)), "cyr_simulate")
+    return out
+
+
+def _stream_ptr(stream):
+    if stream is None:
+        import torch
+        stream = torch.cuda.current_stream()
+    return ctypes.c_void_p(stream.cuda_stream)
+
+
+def simulate_device(sim: Sim, d_voxels, mats, d_out, stream=None):
+    """Device API over torch tensors (inputs resident in HBM)."""
+    m = np.ascontiguousarray(mats, dtype=np.float64)
+    check(lib().cyr_simulate_device(sim.ref, ctypes.c_void_p(d_voxels.data_ptr()), _ptr(m),
+                                    ctypes.c_void_p(d_out.data_ptr()), _stream_ptr(stream)), "cyr_simulate_device")
+
+
+def polarization(sim: Sim, voxels, mats, energy_id, eangle_rad):
+    rd = sim.real_dtype
+    cd = np.complex128 if rd == np.float64 else np.complex64
+    vox = np.ascontiguousarray(voxels, dtype=rd)
+    m = np.ascontiguousarray(mats, dtype=np.float64)
+    nx, ny, nz = sim.dims
+    V = nx * ny * nz
+    px, py, pz = (np.empty(V, dtype=cd) for _ in range(3))
+    check(lib().cyr_polarization(sim.ref, _ptr(vox), _ptr(m), int(energy_id), float(eangle_rad), _ptr(px), _ptr(py),
+                                 _ptr(pz)), "cyr_polarization")
+    return px, py, pz
+
+
+def fft3d_device(vols, dims, precision, stream=None):
+    arr = (ctypes.c_void_p * len(vols))(*[v.data_ptr() for v in vols])
+    nx, ny, nz = dims
+    check(lib().cyr_fft3d_device(nx, ny, nz, precision, arr, len(vols), _stream_ptr(stream)), "cyr_fft3d_device")
+
+
+def dc_shift_device(vol, dims, precision, do_dc=True, do_shift=True, stream=None):
+    nx, ny, nz = dims
+    check(lib().cyr_dc_shift_device(nx, ny, nz, precision, ctypes.c_void_p(vol.data_ptr()), int(do_dc),
+                                    int(do_shift), _stream_ptr(stream)), "cyr_dc_shift_device")
+
+
+def ewald_device(sim: Sim, energy_id, k_id, px, py, pz, proj, stream=None):
+    check(lib().cyr_ewald_device(sim.ref, energy_id, k_id, ctypes.c_void_p(px.data_ptr()),
+                                 ctypes.c_void_p(py.data_ptr()), ctypes.c_void_p(pz.data_ptr()),
+                                 ctypes.c_void_p(proj.data_ptr()), _stream_ptr(stream)), "cyr_ewald_device")
+
+
+def warp_affine_device(src, dst, coeffs, precision, stream=None):
+    ny, nx = src.shape[-2:]
+    c = np.ascontiguousarray(coeffs, dtype=np.float64)
+    check(lib().cyr_warp_affine_device(nx, ny, precision, ctypes.c_void_p(src.data_ptr()),
+                                       ctypes.c_void_p(dst.data_ptr()), _ptr(c), _stream_ptr(stream)),
+          "cyr_warp_affine_device")
+
+
+def simulate_batch(sims, d_voxels, mats, d_outs, stream=None):
+    n = len(sims)
+    descs = (SimDesc * n)(*[s.desc for s in sims])
+    mats_c = [np.ascontiguousarray(m, dtype=np.float64) for m in mats]
+    vp = (ctypes.c_void_p * n)(*[v.data_ptr() for v in d_voxels])
+    mp = (ctypes.c_void_p * n)(*[m.ctypes.data for m in mats_c])
+    op = (ctypes.c_void_p * n)(*[o.data_ptr() for o in d_outs])
+    check(lib().cyr_simulate_batch(n, descs, vp, mp, op, _stream_ptr(stream)), "cyr_simulate_batch")
+
+
+def device_count():
+    return int(lib().cyr_device_count())
+
+
+def version():
+    return lib().cyr_version().decode()

@@ -1,0 +1,124 @@
+// Shared types, error plumbing and device helpers of the CyRSoXS HIP library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <hip/hip_bf16.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/cyrsoxs.h"
+
+namespace cyr {
+
+// ---------------------------------------------------------------------------
+// Error plumbing: every entry point returns a cyr_status; the detail string is
+// kept per thread for cyr_last_error().
+// ---------------------------------------------------------------------------
+void set_error(const std::string &msg);
+cyr_status fail(cyr_status s, const std::string &msg);
+cyr_status hip_fail(hipError_t e, const char *what, const char *file, int line);
+
+#define CYR_HIP(expr)                                                   \
+  do {                                                                  \
+    hipError_t _e = (expr);                                             \
+    if (_e != hipSuccess) return ::cyr::hip_fail(_e, #expr, __FILE__, __LINE__); \
+  } while (0)
+
+#define CYR_TRY(expr)                     \
+  do {                                    \
+    cyr_status _s = (expr);               \
+    if (_s != CYR_OK) return _s;          \
+  } while (0)
+
+// Checks that the last kernel launch was accepted.
+#define CYR_LAUNCHED() CYR_HIP(hipGetLastError())
+
+// ---------------------------------------------------------------------------
+// Complex types: interleaved (re, im) pairs of the compute precision.
+// ---------------------------------------------------------------------------
+template <typename T> struct CplxOf;
+template <> struct CplxOf<float> { using type = float2; };
+template <> struct CplxOf<double> { using type = double2; };
+template <typename T> using cx = typename CplxOf<T>::type;
+
+template <typename T> __host__ __device__ inline cx<T> mk(T re, T im) {
+  cx<T> r;
+  r.x = re;
+  r.y = im;
+  return r;
+}
+template <typename T> __device__ inline cx<T> cadd(cx<T> a, cx<T> b) { return mk<T>(a.x + b.x, a.y + b.y); }
+template <typename T> __device__ inline cx<T> csub(cx<T> a, cx<T> b) { return mk<T>(a.x - b.x, a.y - b.y); }
+template <typename T> __device__ inline cx<T> cmul(cx<T> a, cx<T> b) {
+  return mk<T>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <typename T> __device__ inline cx<T> cscale(cx<T> a, T s) { return mk<T>(a.x * s, a.y * s); }
+// multiply by -i
+template <typename T> __device__ inline cx<T> cmul_mi(cx<T> a) { return mk<T>(a.y, -a.x); }
+
+// ---------------------------------------------------------------------------
+// Morphology storage: 4 values per voxel per material.
+// ---------------------------------------------------------------------------
+struct alignas(8) half4 { __half x, y, z, w; };
+struct alignas(8) bf164 { __hip_bfloat16 x, y, z, w; };
+
+template <typename T, typename S> struct VoxLoad;
+template <typename T> struct VoxLoad<T, float4> {
+  __device__ static inline void get(const float4 *p, uint64_t i, T v[4]) {
+    float4 a = p[i];
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+  }
+};
+template <typename T> struct VoxLoad<T, double4> {
+  __device__ static inline void get(const double4 *p, uint64_t i, T v[4]) {
+    double4 a = p[i];
+    v[0] = (T)a.x; v[1] = (T)a.y; v[2] = (T)a.z; v[3] = (T)a.w;
+  }
+};
+template <typename T> struct VoxLoad<T, half4> {
+  __device__ static inline void get(const half4 *p, uint64_t i, T v[4]) {
+    half4 a = p[i];
+    v[0] = (T)__half2float(a.x); v[1] = (T)__half2float(a.y);
+    v[2] = (T)__half2float(a.z); v[3] = (T)__half2float(a.w);
+  }
+};
+template <typename T> struct VoxLoad<T, bf164> {
+  __device__ static inline void get(const bf164 *p, uint64_t i, T v[4]) {
+    bf164 a = p[i];
+    v[0] = (T)__bfloat162float(a.x); v[1] = (T)__bfloat162float(a.y);
+    v[2] = (T)__bfloat162float(a.z); v[3] = (T)__bfloat162float(a.w);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Per-material optical constants for one energy, derived on the host in FP64:
+//   a = npara^2 - 1, b = nperp^2 - 1, c = (npara + 2 nperp)^2 / 9 - 1,
+//   d = npara^2 - nperp^2   (complex, stored re/im).
+// The vacuum term (the "- Vfrac" of include/uniaxial.h:106-150) is folded in
+// analytically so FP32 does not lose three digits to 1 - n^2 cancellation.
+// ---------------------------------------------------------------------------
+template <typename T> struct MatConst {
+  T ar, ai, br, bi, cr, ci, dr, di;
+};
+
+// Device-side 3x3 real matrix and 3-vector (row-major).
+template <typename T> struct Mat3 { T m[9]; };
+template <typename T> struct Vec3 { T x, y, z; };
+
+inline unsigned ceil_div(uint64_t a, uint64_t b) { return (unsigned)((a + b - 1) / b); }
+
+// ---------------------------------------------------------------------------
+// Per-device cached work buffers (grown on demand, freed by
+// cyr_release_workspace or at process exit).
+// ---------------------------------------------------------------------------
+struct DeviceBuffer {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+};
+cyr_status ensure_buffer(DeviceBuffer &b, size_t bytes);
+void free_buffer(DeviceBuffer &b);
+
+}  // namespace cyr

@@ -1,0 +1,499 @@
+// Forward complex-to-complex FFT pencils for gfx950 (replaces the reference's
+// cufftExecC2C/Z2Z calls, include/cudaMain.h:184-190, src/cudaMain.cu:668-670).
+//
+// Contract: F[k] = sum_n x[n] exp(-2 pi i n k / N) along one axis, unnormalised,
+// in place.  A 3D transform is three axis passes (X, Y, Z).
+//
+// Design (wave64 / LDS):
+//   * A block owns `pp` whole pencils.  For the contiguous X axis the threads of a
+//     pencil are consecutive lanes (row loads are unit-stride); for the strided
+//     Y/Z axes the lanes run across `pp` adjacent pencils (each element load is a
+//     pp*sizeof(complex) contiguous segment).
+//   * Each thread holds E elements in registers; every Stockham stage is
+//     radix R in {2,3,4,5,8} with E/R butterflies per thread.  Stage 0 reads
+//     straight from HBM and the last stage writes straight to HBM (both in the
+//     Stockham access order, which is coalesced); intermediate stages go through
+//     LDS.  Each element is read once and written once per pass.
+//   * Twiddles come from a per-length table generated in FP64 on the host.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+#include "fft.hpp"
+
+namespace cyr {
+
+namespace {
+
+constexpr int kMaxStages = 16;
+
+struct StagePlan {
+  uint32_t n;
+  uint32_t nstages;
+  uint32_t radix[kMaxStages];
+  uint32_t ls[kMaxStages];
+  uint32_t tstep[kMaxStages];
+};
+
+struct PassArgs {
+  void *vol[8];
+  uint64_t elem_stride;
+  uint64_t inner_stride;
+  uint64_t outer_stride;
+  uint32_t inner_count;
+  uint32_t outer_count;
+  uint32_t groups;  // ceil(inner_count / pp)
+  uint32_t pp;
+  uint32_t tpp;
+};
+
+// ---- radix butterflies (forward, W = exp(-2 pi i / R)) ----------------------
+
+template <typename T> __device__ inline void bfly2(cx<T> *a) {
+  cx<T> t = a[0];
+  a[0] = cadd<T>(t, a[1]);
+  a[1] = csub<T>(t, a[1]);
+}
+
+template <typename T> __device__ inline void bfly4(cx<T> *a) {
+  cx<T> t0 = cadd<T>(a[0], a[2]), t1 = csub<T>(a[0], a[2]);
+  cx<T> t2 = cadd<T>(a[1], a[3]), t3 = cmul_mi<T>(csub<T>(a[1], a[3]));
+  a[0] = cadd<T>(t0, t2);
+  a[1] = cadd<T>(t1, t3);
+  a[2] = csub<T>(t0, t2);
+  a[3] = csub<T>(t1, t3);
+}
+
+template <typename T> __device__ inline void bfly8(cx<T> *a) {
+  const T r = (T)0.70710678118654752440084436210484903928;
+  cx<T> e[4] = {a[0], a[2], a[4], a[6]};
+  cx<T> o[4] = {a[1], a[3], a[5], a[7]};
+  bfly4<T>(e);
+  bfly4<T>(o);
+  // W8^1 = (1 - i)/sqrt2, W8^2 = -i, W8^3 = -(1 + i)/sqrt2
+  cx<T> o1 = mk<T>((o[1].x + o[1].y) * r, (o[1].y - o[1].x) * r);
+  cx<T> o2 = cmul_mi<T>(o[2]);
+  cx<T> o3 = mk<T>((o[3].y - o[3].x) * r, -(o[3].x + o[3].y) * r);
+  a[0] = cadd<T>(e[0], o[0]);
+  a[4] = csub<T>(e[0], o[0]);
+  a[1] = cadd<T>(e[1], o1);
+  a[5] = csub<T>(e[1], o1);
+  a[2] = cadd<T>(e[2], o2);
+  a[6] = csub<T>(e[2], o2);
+  a[3] = cadd<T>(e[3], o3);
+  a[7] = csub<T>(e[3], o3);
+}
+
+template <typename T> __device__ inline void bfly3(cx<T> *a) {
+  const T s = (T)0.86602540378443864676372317075293618347;  // sin(2 pi/3)
+  cx<T> t1 = cadd<T>(a[1], a[2]);
+  cx<T> t2 = csub<T>(a[1], a[2]);
+  cx<T> m = mk<T>(a[0].x - (T)0.5 * t1.x, a[0].y - (T)0.5 * t1.y);
+  cx<T> u = mk<T>(s * t2.y, -s * t2.x);  // -i s t2
+  a[0] = cadd<T>(a[0], t1);
+  a[1] = cadd<T>(m, u);
+  a[2] = csub<T>(m, u);
+}
+
+template <typename T> __device__ inline void bfly5(cx<T> *a) {
+  const T c1 = (T)0.30901699437494742410229341718281905886;   // cos(2 pi/5)
+  const T c2 = (T)-0.80901699437494742410229341718281905886;  // cos(4 pi/5)
+  const T s1 = (T)0.95105651629515357211643933337938214340;   // sin(2 pi/5)
+  const T s2 = (T)0.58778525229247312916870595463907276860;   // sin(4 pi/5)
+  cx<T> b1 = cadd<T>(a[1], a[4]), b2 = cadd<T>(a[2], a[3]);
+  cx<T> d1 = csub<T>(a[1], a[4]), d2 = csub<T>(a[2], a[3]);
+  cx<T> m1 = mk<T>(a[0].x + c1 * b1.x + c2 * b2.x, a[0].y + c1 * b1.y + c2 * b2.y);
+  cx<T> m2 = mk<T>(a[0].x + c2 * b1.x + c1 * b2.x, a[0].y + c2 * b1.y + c1 * b2.y);
+  cx<T> n1 = mk<T>(s1 * d1.x + s2 * d2.x, s1 * d1.y + s2 * d2.y);
+  cx<T> n2 = mk<T>(s2 * d1.x - s1 * d2.x, s2 * d1.y - s1 * d2.y);
+  a[0] = mk<T>(a[0].x + b1.x + b2.x, a[0].y + b1.y + b2.y);
+  // y = m -/+ i n
+  a[1] = mk<T>(m1.x + n1.y, m1.y - n1.x);
+  a[4] = mk<T>(m1.x - n1.y, m1.y + n1.x);
+  a[2] = mk<T>(m2.x + n2.y, m2.y - n2.x);
+  a[3] = mk<T>(m2.x - n2.y, m2.y + n2.x);
+}
+
+template <typename T, int R> __device__ inline void bfly(cx<T> *a) {
+  if constexpr (R == 2) bfly2<T>(a);
+  else if constexpr (R == 3) bfly3<T>(a);
+  else if constexpr (R == 4) bfly4<T>(a);
+  else if constexpr (R == 5) bfly5<T>(a);
+  else bfly8<T>(a);
+}
+
+// ---- one Stockham stage ------------------------------------------------------
+//
+// Butterfly j (0 <= j < n/R) of a stage with sub-transform length Ls reads
+// x[j + r n/R], multiplies by W_{Ls R}^{r (j mod Ls)}, does an R-point DFT and
+// writes y[(j / Ls) Ls R + (j mod Ls) + r Ls].
+
+template <typename T, int E, int R, bool CONTIG>
+__device__ inline void run_stage(cx<T> (&v)[E], const StagePlan &P, uint32_t s, bool first, bool last,
+                                 uint32_t t, uint32_t p, const PassArgs &A, cx<T> *gbase, bool active,
+                                 cx<T> *lds, uint32_t lpitch, const cx<T> *__restrict__ tw) {
+  constexpr int NB = E / R;
+  const uint32_t n = P.n;
+  const uint32_t m = n / R;
+  const uint32_t Ls = P.ls[s];
+  const uint32_t tstep = P.tstep[s];
+  const uint32_t tpp = A.tpp;
+  // ---- gather inputs
+#pragma unroll
+  for (int b = 0; b < NB; b++) {
+    const uint32_t j = t + b * tpp;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const uint32_t idx = j + r * m;
+      if (first) {
+        v[b * R + r] = active ? gbase[(uint64_t)idx * A.elem_stride] : mk<T>(0, 0);
+      } else {
+        v[b * R + r] = CONTIG ? lds[p * lpitch + idx] : lds[idx * lpitch + p];
+      }
+    }
+  }
+  if (!first) __syncthreads();
+  // ---- twiddle and butterfly
+#pragma unroll
+  for (int b = 0; b < NB; b++) {
+    const uint32_t j = t + b * tpp;
+    const uint32_t k = j % Ls;
+    if (k != 0) {
+#pragma unroll
+      for (int r = 1; r < R; r++) v[b * R + r] = cmul<T>(v[b * R + r], tw[r * k * tstep]);
+    }
+    bfly<T, R>(&v[b * R]);
+  }
+  // ---- scatter outputs
+#pragma unroll
+  for (int b = 0; b < NB; b++) {
+    const uint32_t j = t + b * tpp;
+    const uint32_t k = j % Ls;
+    const uint32_t o0 = (j / Ls) * Ls * R + k;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const uint32_t o = o0 + r * Ls;
+      if (last) {
+        if (active) gbase[(uint64_t)o * A.elem_stride] = v[b * R + r];
+      } else {
+        if (CONTIG) lds[p * lpitch + o] = v[b * R + r];
+        else lds[o * lpitch + p] = v[b * R + r];
+      }
+    }
+  }
+  if (!last) __syncthreads();
+}
+
+template <typename T, int E, bool CONTIG>
+__global__ void __launch_bounds__(1024) fft_pencil_kernel(StagePlan P, PassArgs A,
+                                                          const cx<T> *__restrict__ tw) {
+  extern __shared__ unsigned char smem_raw[];
+  cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
+  const uint32_t tid = threadIdx.x;
+  uint32_t t, p;
+  if (CONTIG) {
+    t = tid % A.tpp;
+    p = tid / A.tpp;
+  } else {
+    p = tid % A.pp;
+    t = tid / A.pp;
+  }
+  const uint32_t g = blockIdx.x % A.groups;
+  const uint32_t o = blockIdx.x / A.groups;
+  const uint32_t pencil = g * A.pp + p;
+  const bool active = pencil < A.inner_count;
+  cx<T> *gbase = reinterpret_cast<cx<T> *>(A.vol[blockIdx.y]) + (uint64_t)pencil * A.inner_stride +
+                 (uint64_t)o * A.outer_stride;
+  const uint32_t lpitch = CONTIG ? P.n : A.pp;
+  cx<T> v[E];
+  for (uint32_t s = 0; s < P.nstages; s++) {
+    const bool first = s == 0, last = s + 1 == P.nstages;
+    switch (P.radix[s]) {
+      case 2:
+        if constexpr (E % 2 == 0) run_stage<T, E, 2, CONTIG>(v, P, s, first, last, t, p, A, gbase, active, lds, lpitch, tw);
+        break;
+      case 3:
+        if constexpr (E % 3 == 0) run_stage<T, E, 3, CONTIG>(v, P, s, first, last, t, p, A, gbase, active, lds, lpitch, tw);
+        break;
+      case 4:
+        if constexpr (E % 4 == 0) run_stage<T, E, 4, CONTIG>(v, P, s, first, last, t, p, A, gbase, active, lds, lpitch, tw);
+        break;
+      case 5:
+        if constexpr (E % 5 == 0) run_stage<T, E, 5, CONTIG>(v, P, s, first, last, t, p, A, gbase, active, lds, lpitch, tw);
+        break;
+      default:
+        if constexpr (E % 8 == 0) run_stage<T, E, 8, CONTIG>(v, P, s, first, last, t, p, A, gbase, active, lds, lpitch, tw);
+        break;
+    }
+  }
+}
+
+// Direct O(n^2) DFT of each pencil for lengths with prime factors > 5: the
+// pencil is staged in LDS, then every thread produces outputs.
+template <typename T>
+__global__ void __launch_bounds__(256) dft_pencil_kernel(uint32_t n, PassArgs A, const cx<T> *__restrict__ tw) {
+  extern __shared__ unsigned char smem_raw[];
+  cx<T> *x = reinterpret_cast<cx<T> *>(smem_raw);
+  const uint32_t g = blockIdx.x % A.inner_count;
+  const uint32_t o = blockIdx.x / A.inner_count;
+  cx<T> *base = reinterpret_cast<cx<T> *>(A.vol[blockIdx.y]) + (uint64_t)g * A.inner_stride +
+                (uint64_t)o * A.outer_stride;
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) x[i] = base[(uint64_t)i * A.elem_stride];
+  __syncthreads();
+  for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) {
+    T re = 0, im = 0;
+    uint32_t e = 0;
+    for (uint32_t i = 0; i < n; i++) {
+      const cx<T> w = tw[e];
+      re += x[i].x * w.x - x[i].y * w.y;
+      im += x[i].x * w.y + x[i].y * w.x;
+      e += k;
+      if (e >= n) e -= n;
+    }
+    base[(uint64_t)k * A.elem_stride] = mk<T>(re, im);
+  }
+}
+
+// ---- host-side planning ------------------------------------------------------
+
+struct Factoring {
+  bool ok = false;
+  std::vector<uint32_t> radices;
+  uint32_t lcm = 1;
+};
+
+Factoring factor(uint32_t n) {
+  Factoring f;
+  uint32_t r = n, a = 0, b = 0, c = 0;
+  while (r % 2 == 0) { r /= 2; a++; }
+  while (r % 3 == 0) { r /= 3; b++; }
+  while (r % 5 == 0) { r /= 5; c++; }
+  if (r != 1) return f;
+  while (a >= 3) { f.radices.push_back(8); a -= 3; }
+  uint32_t p2 = 1;
+  for (uint32_t rr : f.radices) p2 = rr > p2 ? rr : p2;
+  if (a == 2) { f.radices.push_back(4); p2 = p2 > 4 ? p2 : 4; }
+  if (a == 1) { f.radices.push_back(2); p2 = p2 > 2 ? p2 : 2; }
+  for (uint32_t i = 0; i < b; i++) f.radices.push_back(3);
+  for (uint32_t i = 0; i < c; i++) f.radices.push_back(5);
+  f.lcm = p2 * (b ? 3 : 1) * (c ? 5 : 1);
+  f.ok = f.radices.size() <= (size_t)kMaxStages;
+  return f;
+}
+
+std::mutex g_mu;
+std::map<std::tuple<int, uint32_t, int>, DeviceBuffer> g_tw;  // (device, n, precision)
+std::map<std::tuple<uint32_t, int, int>, FftShape> g_shapes;   // (n, contig, precision)
+
+template <typename T> constexpr int prec_of() { return sizeof(T) == 8 ? CYR_FP64 : CYR_FP32; }
+
+template <typename T, bool CONTIG>
+hipError_t launch_e(uint32_t E, dim3 grid, dim3 block, size_t lds, hipStream_t st, const StagePlan &P,
+                    const PassArgs &A, const cx<T> *tw, bool *found) {
+  *found = true;
+#define CYR_FFT_CASE(EV)                                                                     \
+  case EV:                                                                                   \
+    hipLaunchKernelGGL((fft_pencil_kernel<T, EV, CONTIG>), grid, block, lds, st, P, A, tw); \
+    return hipGetLastError();
+  switch (E) {
+    CYR_FFT_CASE(2)
+    CYR_FFT_CASE(3)
+    CYR_FFT_CASE(4)
+    CYR_FFT_CASE(5)
+    CYR_FFT_CASE(6)
+    CYR_FFT_CASE(8)
+    CYR_FFT_CASE(10)
+    CYR_FFT_CASE(12)
+    CYR_FFT_CASE(16)
+    CYR_FFT_CASE(20)
+    CYR_FFT_CASE(24)
+    CYR_FFT_CASE(32)
+    CYR_FFT_CASE(40)
+    default:
+      *found = false;
+      return hipSuccess;
+  }
+#undef CYR_FFT_CASE
+}
+
+size_t max_lds_bytes() {
+  static size_t v = 0;
+  if (!v) {
+    int dev = 0;
+    hipGetDevice(&dev);
+    int b = 0;
+    if (hipDeviceGetAttribute(&b, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || b <= 0)
+      b = 65536;
+    v = (size_t)b;
+  }
+  return v;
+}
+
+}  // namespace
+
+template <typename T> const cx<T> *twiddles(uint32_t n, cyr_status *st) {
+  int dev = 0;
+  hipGetDevice(&dev);
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto key = std::make_tuple(dev, n, prec_of<T>());
+  auto it = g_tw.find(key);
+  if (it != g_tw.end()) {
+    *st = CYR_OK;
+    return reinterpret_cast<const cx<T> *>(it->second.ptr);
+  }
+  std::vector<cx<T>> h(n);
+  for (uint32_t m = 0; m < n; m++) {
+    // exact-angle FP64 twiddle, rounded once to the compute precision
+    const double a = 2.0 * M_PI * (double)m / (double)n;
+    h[m].x = (T)std::cos(a);
+    h[m].y = (T)(-std::sin(a));
+  }
+  DeviceBuffer b;
+  cyr_status s = ensure_buffer(b, sizeof(cx<T>) * n);
+  if (s != CYR_OK) {
+    *st = s;
+    return nullptr;
+  }
+  if (hipMemcpy(b.ptr, h.data(), sizeof(cx<T>) * n, hipMemcpyHostToDevice) != hipSuccess) {
+    free_buffer(b);
+    *st = fail(CYR_E_DEVICE, "twiddle upload failed");
+    return nullptr;
+  }
+  g_tw[key] = b;
+  *st = CYR_OK;
+  return reinterpret_cast<const cx<T> *>(b.ptr);
+}
+
+void fft_set_shape(uint32_t n, int contig, int precision, FftShape s) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_shapes[std::make_tuple(n, contig, precision)] = s;
+}
+
+void fft_clear_shapes() {
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_shapes.clear();
+}
+
+void fft_release() {
+  std::lock_guard<std::mutex> lk(g_mu);
+  for (auto &kv : g_tw) free_buffer(kv.second);
+  g_tw.clear();
+}
+
+template <typename T>
+cyr_status fft_axis(void *const *vols, uint32_t count, uint32_t nx, uint32_t ny, uint32_t nz, int axis,
+                    hipStream_t stream) {
+  const uint32_t n = axis == 0 ? nx : axis == 1 ? ny : nz;
+  if (n <= 1 || count == 0) return CYR_OK;
+  cyr_status st;
+  const cx<T> *tw = twiddles<T>(n, &st);
+  if (!tw) return st;
+  PassArgs A{};
+  const bool contig = axis == 0;
+  if (axis == 0) {
+    A.elem_stride = 1;
+    A.inner_stride = nx;
+    A.inner_count = ny;
+    A.outer_stride = (uint64_t)nx * ny;
+    A.outer_count = nz;
+  } else if (axis == 1) {
+    A.elem_stride = nx;
+    A.inner_stride = 1;
+    A.inner_count = nx;
+    A.outer_stride = (uint64_t)nx * ny;
+    A.outer_count = nz;
+  } else {
+    A.elem_stride = (uint64_t)nx * ny;
+    A.inner_stride = 1;
+    A.inner_count = nx;
+    A.outer_stride = nx;
+    A.outer_count = ny;
+  }
+  const Factoring f = factor(n);
+  const size_t lds_max = max_lds_bytes();
+  for (uint32_t c0 = 0; c0 < count; c0 += 8) {
+    const uint32_t nc = count - c0 < 8 ? count - c0 : 8;
+    for (uint32_t i = 0; i < nc; i++) A.vol[i] = vols[c0 + i];
+    bool done = false;
+    if (f.ok) {
+      // Elements per thread: the radix lcm, doubled while a pencil would need more
+      // than 256 threads; pencils per block: up to 256 threads (contiguous) or a
+      // 16-pencil coalescing group (strided), bounded by LDS.
+      FftShape shp;
+      {
+        std::lock_guard<std::mutex> lk(g_mu);
+        auto it = g_shapes.find(std::make_tuple(n, (int)contig, prec_of<T>()));
+        if (it != g_shapes.end()) shp = it->second;
+      }
+      uint32_t E = f.lcm;
+      const uint32_t tpp_cap = contig ? 256 : 32;  // strided: 16 pencils x 32 threads
+      const uint32_t e_cap = sizeof(T) == 8 ? 16 : 32;
+      if (shp.e && shp.e % f.lcm == 0 && n % shp.e == 0) E = shp.e;
+      else
+        while (n / E > tpp_cap && (E * 2) <= e_cap && n % (E * 2) == 0) E *= 2;
+      if (n % E == 0 && n >= E) {
+        const uint32_t tpp = n / E;
+        uint32_t pp;
+        if (shp.pp) pp = shp.pp;
+        else if (contig) pp = tpp >= 256 ? 1 : 256 / tpp;
+        else pp = 16;
+        while (pp > 1 && ((size_t)pp * n * sizeof(cx<T>) > lds_max || pp * tpp > 1024)) pp /= 2;
+        if (contig && pp > A.inner_count) pp = A.inner_count;
+        if ((size_t)pp * n * sizeof(cx<T>) <= lds_max && pp * tpp <= 1024) {
+          StagePlan P{};
+          P.n = n;
+          P.nstages = (uint32_t)f.radices.size();
+          uint32_t Ls = 1;
+          for (uint32_t s = 0; s < P.nstages; s++) {
+            P.radix[s] = f.radices[s];
+            P.ls[s] = Ls;
+            P.tstep[s] = n / (Ls * f.radices[s]);
+            Ls *= f.radices[s];
+          }
+          A.pp = pp;
+          A.tpp = tpp;
+          A.groups = (A.inner_count + pp - 1) / pp;
+          const uint64_t blocks = (uint64_t)A.groups * A.outer_count;
+          if (blocks > 0x7fffffffull) return fail(CYR_E_UNSUPPORTED, "FFT grid too large");
+          dim3 grid((unsigned)blocks, nc), block(pp * tpp);
+          const size_t lds = (size_t)pp * n * sizeof(cx<T>);
+          bool found = false;
+          hipError_t e = contig ? launch_e<T, true>(E, grid, block, lds, stream, P, A, tw, &found)
+                                : launch_e<T, false>(E, grid, block, lds, stream, P, A, tw, &found);
+          if (e != hipSuccess) return hip_fail(e, "fft_pencil_kernel", __FILE__, __LINE__);
+          done = found;
+        }
+      }
+    }
+    if (!done) {
+      if ((size_t)n * sizeof(cx<T>) > lds_max) return fail(CYR_E_UNSUPPORTED, "FFT length not supported");
+      const uint64_t blocks = (uint64_t)A.inner_count * A.outer_count;
+      if (blocks > 0x7fffffffull) return fail(CYR_E_UNSUPPORTED, "FFT grid too large");
+      dim3 grid((unsigned)blocks, nc), block(256);
+      hipLaunchKernelGGL((dft_pencil_kernel<T>), grid, block, (size_t)n * sizeof(cx<T>), stream, n, A, tw);
+      CYR_LAUNCHED();
+    }
+  }
+  return CYR_OK;
+}
+
+template <typename T>
+cyr_status fft3d(void *const *vols, uint32_t count, uint32_t nx, uint32_t ny, uint32_t nz, hipStream_t stream) {
+  CYR_TRY(fft_axis<T>(vols, count, nx, ny, nz, 0, stream));
+  CYR_TRY(fft_axis<T>(vols, count, nx, ny, nz, 1, stream));
+  CYR_TRY(fft_axis<T>(vols, count, nx, ny, nz, 2, stream));
+  return CYR_OK;
+}
+
+template const cx<float> *twiddles<float>(uint32_t, cyr_status *);
+template const cx<double> *twiddles<double>(uint32_t, cyr_status *);
+template cyr_status fft_axis<float>(void *const *, uint32_t, uint32_t, uint32_t, uint32_t, int, hipStream_t);
+template cyr_status fft_axis<double>(void *const *, uint32_t, uint32_t, uint32_t, uint32_t, int, hipStream_t);
+template cyr_status fft3d<float>(void *const *, uint32_t, uint32_t, uint32_t, uint32_t, hipStream_t);
+template cyr_status fft3d<double>(void *const *, uint32_t, uint32_t, uint32_t, uint32_t, hipStream_t);
+
+}  // namespace cyr

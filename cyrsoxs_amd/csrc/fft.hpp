@@ -1,0 +1,34 @@
+// Hand-written forward C2C FFT for gfx950: Stockham autosort pencils staged in
+// LDS, one launch per axis for a batch of volumes.
+#pragma once
+
+#include <cstdint>
+
+#include "common.hpp"
+
+namespace cyr {
+
+// One axis pass over `count` volumes laid out X-fastest with dims (nx, ny, nz).
+// axis 0 = X (contiguous pencils), 1 = Y, 2 = Z (strided pencils).
+template <typename T>
+cyr_status fft_axis(void *const *vols, uint32_t count, uint32_t nx, uint32_t ny, uint32_t nz,
+                    int axis, hipStream_t stream);
+
+// Full forward unnormalised 3D FFT (X, then Y, then Z) of `count` volumes.
+template <typename T>
+cyr_status fft3d(void *const *vols, uint32_t count, uint32_t nx, uint32_t ny, uint32_t nz,
+                 hipStream_t stream);
+
+// Twiddle table W_n[m] = exp(-2 pi i m / n), m < n, generated in FP64 on the
+// host and cached per device (stored in the compute precision).
+template <typename T> const cx<T> *twiddles(uint32_t n, cyr_status *st);
+
+// Launch-shape override for the tuner: elements per thread and pencils per block
+// for an axis length (0 = automatic choice).
+struct FftShape { uint32_t e = 0, pp = 0; };
+void fft_set_shape(uint32_t n, int contig, int precision, FftShape s);
+void fft_clear_shapes();
+
+void fft_release();
+
+}  // namespace cyr

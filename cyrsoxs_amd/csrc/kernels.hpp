@@ -1,0 +1,99 @@
+// Kernel argument blocks and launchers shared by the executors.
+#pragma once
+
+#include <cstdint>
+
+#include "common.hpp"
+
+namespace cyr {
+
+// ---- polarization (polarization.hip) ----------------------------------------
+template <typename T> struct PolarArgs {
+  const MatConst<T> *mats;  // this energy's M constants (device)
+  uint32_t M;
+  uint64_t V;
+  uint32_t nx, ny, nz;
+  uint32_t rows, block, blocks_per_row;  // filled by the launcher
+  const T *hx, *hy, *hz;                 // Hanning tables (device) or null
+  Vec3<T> e[2];                          // E basis vectors
+  Mat3<T> R;                             // E rotation (MATERIAL frame)
+  cx<T> *out[6];                         // [basis][component] volumes
+};
+
+template <typename T>
+cyr_status launch_polarization(PolarArgs<T> a, const void *vox, int storage, int morph, int nb, bool hann,
+                               bool matframe, hipStream_t st);
+template <typename T>
+cyr_status launch_nt(PolarArgs<T> a, const void *vox, int storage, int morph, cx<T> *nt, hipStream_t st);
+template <typename T>
+cyr_status launch_polarization_nt(PolarArgs<T> a, const cx<T> *nt, int nb, bool matframe, hipStream_t st);
+
+// ---- Ewald projection (ewald.hip) -------------------------------------------
+// Everything geometric is FP64: q grid, Ewald validity, z index and weight.
+struct EwaldGeom {
+  uint32_t nx, ny, nz;
+  double start, dx, dy, dz;  // q grid (include/uniaxial.h:842-856)
+  double k;                  // |k| for this energy
+  double kx, ky, kz;         // unit k vector
+  int interpolation;         // 0 NN, 1 linear
+  int enable2d;
+};
+EwaldGeom make_ewald_geom(uint32_t nx, uint32_t ny, uint32_t nz, double phys_size, double k,
+                          const double kvec[3], int interpolation);
+
+// Plain path: projection of one E angle from the three UNshifted FFT volumes.
+template <typename T>
+cyr_status launch_ewald_partial(const EwaldGeom &g, const cx<T> *px, const cx<T> *py, const cx<T> *pz, T *proj,
+                                hipStream_t st);
+// Full approach: I over the whole shifted volume, then the projection.
+template <typename T>
+cyr_status launch_scatter3d(const EwaldGeom &g, const cx<T> *px, const cx<T> *py, const cx<T> *pz, T *scatter,
+                            hipStream_t st);
+template <typename T>
+cyr_status launch_ewald_full(const EwaldGeom &g, const T *scatter, T *proj, hipStream_t st);
+// DC replacement of F[0] in place for `count` volumes (src/cudaMain.cu:123-176).
+template <typename T>
+cyr_status launch_dc_replace(cx<T> *const *vols, uint32_t count, uint32_t nx, uint32_t ny, uint32_t nz,
+                             hipStream_t st);
+// Stand-alone Igor shift (in-place involution, include/uniaxial.h:570-613).
+template <typename T>
+cyr_status launch_igor_shift(cx<T> *vol, uint32_t nx, uint32_t ny, uint32_t nz, hipStream_t st);
+
+// E-basis path: DC replacement values of the six XY-transformed volumes.
+template <typename T>
+cyr_status launch_ebasis_dc(cx<T> *const *g6, uint32_t nx, uint32_t ny, uint32_t nz, const cx<T> *twz,
+                            cx<T> *dcval, hipStream_t st);
+// E-basis path: pruned z DFT at the Ewald bins + every angle's projection.
+template <typename T> struct EbasisArgs {
+  EwaldGeom g;
+  const cx<T> *vol[6];   // XY-transformed volumes (basis 1 xyz, basis 2 xyz)
+  const cx<T> *twz;      // z twiddles (length nz) or null in 2D
+  const cx<T> *dcval;    // [6] DC replacement values
+  const T *coef;         // [nA][2] E = c e1 + s e2
+  const T *rmat;         // [nA][9] E rotations (MATERIAL frame) or null
+  uint32_t nA;
+  T *proj;               // [nA][Ny][Nx]
+};
+template <typename T> cyr_status launch_ebasis_ewald(const EbasisArgs<T> &a, bool matframe, hipStream_t st);
+
+// ---- rotation / averaging (rotate.hip) --------------------------------------
+// Plain path: rotate one projection by the E angle and accumulate (FP64).
+template <typename T>
+cyr_status launch_rotate_accumulate(const T *proj, double *acc, uint32_t *count, uint32_t nx, uint32_t ny,
+                                    const double inv[6], int rot_mask, hipStream_t st);
+// Average over angles into avg (FP64): mask division or 1/nA.
+cyr_status launch_average(const double *acc, const uint32_t *count, double *avg, uint64_t P, uint32_t nA,
+                          int rot_mask, hipStream_t st);
+// E-basis path: rotate every angle's projection, accumulate and average in one pass.
+template <typename T>
+cyr_status launch_rotate_all(const T *proj, const double *inv_tab, uint32_t nA, uint32_t nx, uint32_t ny,
+                             int rot_mask, double *avg, hipStream_t st);
+// Detector warp of the averaged pattern into the output (prefill 0 or NaN).
+template <typename T>
+cyr_status launch_detector_warp(const double *avg, T *out, uint32_t nx, uint32_t ny, const double inv[6],
+                                int rot_mask, hipStream_t st);
+// Generic bilinear warp (stage-level entry point).
+template <typename T>
+cyr_status launch_warp(const T *src, T *dst, uint32_t nx, uint32_t ny, const double inv[6], hipStream_t st);
+
+}  // namespace cyr

@@ -1,0 +1,567 @@
+// Executors and the whole-simulation / stage-level C ABI.
+//
+// PLAIN (reference structure, src/cudaMain.cu:586-971): for every energy, k
+// vector and E angle: polarization -> 3 full 3D FFTs -> DC replacement ->
+// Ewald projection (shift folded into the gather) -> rotate + mask + FP64
+// accumulate; then average, detector warp, output.
+//
+// EBASIS: p(theta) = c(theta) p1 + s(theta) p2 with p1, p2 the polarizations
+// for the E basis vectors e1 = R_K x^, e2 = k^ x e1 (E(theta) = R(theta) x^ lies
+// in their plane).  Every later stage is linear or a permutation, so per
+// (energy, k) the pipeline computes two polarizations, their XY FFTs, and the z
+// DFT only at each pixel's Ewald bin(s); all angles then become 2D work.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <vector>
+
+#include "fft.hpp"
+#include "host_math.hpp"
+#include "kernels.hpp"
+#include "pipeline.hpp"
+
+namespace cyr {
+
+namespace {
+
+template <typename U>
+__global__ void __launch_bounds__(256) fill_kernel(U *__restrict__ p, U v, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+template <typename U> cyr_status fill(U *p, U v, uint64_t n, hipStream_t st) {
+  if (n == 0) return CYR_OK;
+  hipLaunchKernelGGL((fill_kernel<U>), dim3(ceil_div(n, 256)), dim3(256), 0, st, p, v, n);
+  CYR_LAUNCHED();
+  return CYR_OK;
+}
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// n = (1 - delta) + i beta  ->  n^2 - 1 = delta^2 - 2 delta - beta^2 + 2i (1 - delta) beta
+void sq_minus_1(double delta, double beta, double &re, double &im) {
+  re = delta * delta - 2.0 * delta - beta * beta;
+  im = 2.0 * (1.0 - delta) * beta;
+}
+
+template <typename T> MatConst<T> mat_const(const double *m4) {
+  const double dp = 1.0 - m4[0], bp = m4[1], dq = 1.0 - m4[2], bq = m4[3];
+  double ar, ai, br, bi, cr, ci;
+  sq_minus_1(dp, bp, ar, ai);
+  sq_minus_1(dq, bq, br, bi);
+  sq_minus_1((dp + 2.0 * dq) / 3.0, (bp + 2.0 * bq) / 3.0, cr, ci);
+  MatConst<T> r;
+  r.ar = (T)ar;
+  r.ai = (T)ai;
+  r.br = (T)br;
+  r.bi = (T)bi;
+  r.cr = (T)cr;
+  r.ci = (T)ci;
+  r.dr = (T)(ar - br);
+  r.di = (T)(ai - bi);
+  return r;
+}
+
+// Per-simulation constant tables, uploaded once.
+struct Tables {
+  size_t mats = 0, han = 0, coef = 0, rmat = 0, rotinv = 0, detinv = 0, total = 0;
+  std::vector<unsigned char> host;
+};
+
+template <typename T>
+void build_tables(const cyr_sim_desc *d, const SimGeometry &g, const double *mats, Tables &tb) {
+  size_t off = 0;
+  tb.mats = off;
+  off = align256(off + sizeof(MatConst<T>) * g.nE * g.M);
+  tb.han = off;
+  off = align256(off + sizeof(T) * (g.nx + g.ny + g.nz));
+  tb.coef = off;
+  off = align256(off + sizeof(T) * 2 * g.nK * g.nA);
+  tb.rmat = off;
+  off = align256(off + sizeof(T) * 9 * g.nK * g.nA);
+  tb.rotinv = off;
+  off = align256(off + sizeof(double) * 6 * g.nK * g.nA);
+  tb.detinv = off;
+  off = align256(off + sizeof(double) * 6 * g.nK);
+  tb.total = off;
+  tb.host.assign(off, 0);
+  MatConst<T> *mc = reinterpret_cast<MatConst<T> *>(tb.host.data() + tb.mats);
+  for (uint32_t e = 0; e < g.nE; e++)
+    for (uint32_t m = 0; m < g.M; m++) mc[(size_t)e * g.M + m] = mat_const<T>(mats + 4 * ((size_t)e * g.M + m));
+  // Hanning h(i, N) = 0.5 (1 - cos(2 pi i / N)) (src/cudaMain.cu:283-289); 1 along z in 2D
+  T *h = reinterpret_cast<T *>(tb.host.data() + tb.han);
+  for (uint32_t i = 0; i < g.nx; i++) h[i] = (T)(0.5 * (1 - std::cos(2 * M_PI * i / (g.nx))));
+  for (uint32_t i = 0; i < g.ny; i++) h[g.nx + i] = (T)(0.5 * (1 - std::cos(2 * M_PI * i / (g.ny))));
+  for (uint32_t i = 0; i < g.nz; i++)
+    h[g.nx + g.ny + i] = g.nz == 1 ? (T)1 : (T)(0.5 * (1 - std::cos(2 * M_PI * i / (g.nz))));
+  T *coef = reinterpret_cast<T *>(tb.host.data() + tb.coef);
+  for (size_t i = 0; i < (size_t)2 * g.nK * g.nA; i++) coef[i] = (T)g.ebasis[i];
+  T *rm = reinterpret_cast<T *>(tb.host.data() + tb.rmat);
+  for (size_t i = 0; i < (size_t)g.nK * g.nA; i++)
+    for (int j = 0; j < 9; j++) rm[9 * i + j] = (T)g.R[i].m[j];
+  std::memcpy(tb.host.data() + tb.rotinv, g.rot_inv.data(), sizeof(double) * 6 * g.nK * g.nA);
+  std::memcpy(tb.host.data() + tb.detinv, g.det_inv.data(), sizeof(double) * 6 * g.nK);
+  (void)d;
+}
+
+template <typename T> Vec3<T> v3(const V3 &v) { return Vec3<T>{(T)v.x, (T)v.y, (T)v.z}; }
+template <typename T> Mat3<T> m3(const M3 &m) {
+  Mat3<T> r;
+  for (int i = 0; i < 9; i++) r.m[i] = (T)m.m[i];
+  return r;
+}
+
+template <typename T>
+cyr_status run_impl(const cyr_sim_desc *d, const SimGeometry &g, const void *d_vox, const double *mats, T *d_out,
+                    WorkBuffers &wb, hipStream_t st) {
+  const uint64_t V = g.V, P = g.P;
+  const bool ebasis = d->pipeline == CYR_PIPE_EBASIS;
+  const bool alg1 = d->algorithm == 1;
+  const bool hann = d->windowing == 1 && !alg1;  // Algorithm 1 has no window (include/uniaxial.h:441-521)
+  const bool matframe = d->reference_frame == 0;
+  const bool full = d->scatter_approach == 1 && !ebasis;
+
+  Tables tb;
+  build_tables<T>(d, g, mats, tb);
+  CYR_TRY(ensure_buffer(wb.tables, tb.total));
+  CYR_HIP(hipMemcpyAsync(wb.tables.ptr, tb.host.data(), tb.total, hipMemcpyHostToDevice, st));
+  unsigned char *tbase = reinterpret_cast<unsigned char *>(wb.tables.ptr);
+  const MatConst<T> *d_mats = reinterpret_cast<const MatConst<T> *>(tbase + tb.mats);
+  const T *d_han = reinterpret_cast<const T *>(tbase + tb.han);
+  const T *d_coef = reinterpret_cast<const T *>(tbase + tb.coef);
+  const T *d_rmat = reinterpret_cast<const T *>(tbase + tb.rmat);
+  const double *d_rotinv = reinterpret_cast<const double *>(tbase + tb.rotinv);
+
+  const uint32_t ncomp = ebasis ? 6 : 3;
+  CYR_TRY(ensure_buffer(wb.pol, sizeof(cx<T>) * V * ncomp));
+  cx<T> *pol = reinterpret_cast<cx<T> *>(wb.pol.ptr);
+  cx<T> *vols[6];
+  void *vptr[6];
+  for (uint32_t c = 0; c < ncomp; c++) {
+    vols[c] = pol + (uint64_t)c * V;
+    vptr[c] = vols[c];
+  }
+  if (alg1 && !ebasis) CYR_TRY(ensure_buffer(wb.nt, sizeof(cx<T>) * V * 6));
+  if (full) CYR_TRY(ensure_buffer(wb.scatter, sizeof(T) * V));
+  CYR_TRY(ensure_buffer(wb.proj, sizeof(T) * P * (ebasis ? g.nA : 1)));
+  CYR_TRY(ensure_buffer(wb.acc, sizeof(double) * P));
+  CYR_TRY(ensure_buffer(wb.count, sizeof(uint32_t) * P));
+  CYR_TRY(ensure_buffer(wb.avg, sizeof(double) * P));
+  CYR_TRY(ensure_buffer(wb.dc, sizeof(cx<T>) * 8));
+  T *proj = reinterpret_cast<T *>(wb.proj.ptr);
+  double *acc = reinterpret_cast<double *>(wb.acc.ptr);
+  uint32_t *count = reinterpret_cast<uint32_t *>(wb.count.ptr);
+  double *avg = reinterpret_cast<double *>(wb.avg.ptr);
+  cx<T> *dcval = reinterpret_cast<cx<T> *>(wb.dc.ptr);
+  cyr_status s;
+  const cx<T> *twz = g.nz > 1 ? twiddles<T>(g.nz, &s) : nullptr;
+  if (g.nz > 1 && !twz) return s;
+
+  PolarArgs<T> pa{};
+  pa.M = g.M;
+  pa.V = V;
+  pa.nx = g.nx;
+  pa.ny = g.ny;
+  pa.nz = g.nz;
+  pa.hx = d_han;
+  pa.hy = d_han + g.nx;
+  pa.hz = d_han + g.nx + g.ny;
+  for (uint32_t c = 0; c < ncomp; c++) pa.out[c] = vols[c];
+
+  for (uint32_t e = 0; e < g.nE; e++) {
+    pa.mats = d_mats + (size_t)e * g.M;
+    if (alg1 && !ebasis)
+      CYR_TRY(launch_nt<T>(pa, d_vox, d->storage, d->morphology_type, reinterpret_cast<cx<T> *>(wb.nt.ptr), st));
+    for (uint32_t k = 0; k < g.nK; k++) {
+      const double kv[3] = {g.kvec[k].x, g.kvec[k].y, g.kvec[k].z};
+      const EwaldGeom geo = make_ewald_geom(g.nx, g.ny, g.nz, d->phys_size, g.kmag[e], kv, d->interpolation);
+      T *out = d_out + ((uint64_t)e * g.nK + k) * P;
+      if (ebasis) {
+        pa.e[0] = v3<T>(g.e1[k]);
+        pa.e[1] = v3<T>(g.e2[k]);
+        CYR_TRY(launch_polarization<T>(pa, d_vox, d->storage, d->morphology_type, 2, hann, false, st));
+        CYR_TRY(fft_axis<T>(vptr, 6, g.nx, g.ny, g.nz, 0, st));
+        CYR_TRY(fft_axis<T>(vptr, 6, g.nx, g.ny, g.nz, 1, st));
+        CYR_TRY(launch_ebasis_dc<T>(vols, g.nx, g.ny, g.nz, twz, dcval, st));
+        EbasisArgs<T> ea{};
+        ea.g = geo;
+        for (int c = 0; c < 6; c++) ea.vol[c] = vols[c];
+        ea.twz = twz;
+        ea.dcval = dcval;
+        ea.coef = d_coef + (size_t)2 * k * g.nA;
+        ea.rmat = d_rmat + (size_t)9 * k * g.nA;
+        ea.nA = g.nA;
+        ea.proj = proj;
+        CYR_TRY(launch_ebasis_ewald<T>(ea, matframe, st));
+        CYR_TRY(launch_rotate_all<T>(proj, d_rotinv + (size_t)6 * k * g.nA, g.nA, g.nx, g.ny, d->rot_mask, avg, st));
+      } else {
+        CYR_TRY(fill<double>(acc, 0.0, P, st));
+        CYR_TRY(fill<uint32_t>(count, 0u, P, st));
+        for (uint32_t a = 0; a < g.nA; a++) {
+          const M3 &R = g.R[(size_t)k * g.nA + a];
+          pa.e[0] = Vec3<T>{(T)R.m[0], (T)R.m[3], (T)R.m[6]};  // E = R x^
+          pa.R = m3<T>(R);
+          if (alg1)
+            CYR_TRY(launch_polarization_nt<T>(pa, reinterpret_cast<const cx<T> *>(wb.nt.ptr), 1, matframe, st));
+          else
+            CYR_TRY(launch_polarization<T>(pa, d_vox, d->storage, d->morphology_type, 1, hann, matframe, st));
+          CYR_TRY(fft3d<T>(vptr, 3, g.nx, g.ny, g.nz, st));
+          CYR_TRY(launch_dc_replace<T>(vols, 3, g.nx, g.ny, g.nz, st));
+          if (full) {
+            T *sc = reinterpret_cast<T *>(wb.scatter.ptr);
+            CYR_TRY(launch_scatter3d<T>(geo, vols[0], vols[1], vols[2], sc, st));
+            CYR_TRY(launch_ewald_full<T>(geo, sc, proj, st));
+          } else {
+            CYR_TRY(launch_ewald_partial<T>(geo, vols[0], vols[1], vols[2], proj, st));
+          }
+          CYR_TRY(launch_rotate_accumulate<T>(proj, acc, count, g.nx, g.ny, &g.rot_inv[((size_t)k * g.nA + a) * 6],
+                                              d->rot_mask, st));
+        }
+        CYR_TRY(launch_average(acc, count, avg, P, g.nA, d->rot_mask, st));
+      }
+      CYR_TRY(launch_detector_warp<T>(avg, out, g.nx, g.ny, &g.det_inv[(size_t)k * 6], d->rot_mask, st));
+    }
+  }
+  // keep the host table alive until the upload has been consumed
+  CYR_HIP(hipStreamSynchronize(st));
+  return CYR_OK;
+}
+
+// Per-device state of the plain C ABI: cached work buffers and a library stream.
+struct DeviceState {
+  std::mutex mu;
+  WorkBuffers wb;
+  DeviceBuffer vox, out;
+  hipStream_t stream = nullptr;
+};
+
+std::mutex g_states_mu;
+std::map<int, DeviceState *> g_states;
+
+cyr_status device_state(DeviceState **out) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n < 1) {
+    (void)hipGetLastError();
+    return fail(CYR_E_NODEVICE, "no HIP device found");
+  }
+  int dev = 0;
+  CYR_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_states_mu);
+  auto it = g_states.find(dev);
+  if (it == g_states.end()) {
+    DeviceState *s = new DeviceState();
+    CYR_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    it = g_states.emplace(dev, s).first;
+  }
+  *out = it->second;
+  return CYR_OK;
+}
+
+}  // namespace
+
+void WorkBuffers::release() {
+  free_buffer(pol);
+  free_buffer(nt);
+  free_buffer(scatter);
+  free_buffer(proj);
+  free_buffer(acc);
+  free_buffer(count);
+  free_buffer(avg);
+  free_buffer(tables);
+  free_buffer(dc);
+}
+
+size_t WorkBuffers::bytes() const {
+  return pol.bytes + nt.bytes + scatter.bytes + proj.bytes + acc.bytes + count.bytes + avg.bytes + tables.bytes +
+         dc.bytes;
+}
+
+uint64_t morphology_bytes(const cyr_sim_desc *d) {
+  const uint64_t V = (uint64_t)d->dims[0] * d->dims[1] * d->dims[2];
+  const uint64_t el = d->storage == CYR_STORE_NATIVE ? (d->precision == CYR_FP64 ? 8 : 4) : 2;
+  return V * d->num_materials * 4 * el;
+}
+
+uint64_t output_bytes(const cyr_sim_desc *d, const SimGeometry &g) {
+  return (uint64_t)g.nE * g.nK * g.P * (d->precision == CYR_FP64 ? 8 : 4);
+}
+
+uint64_t work_bytes(const cyr_sim_desc *d, const SimGeometry &g) {
+  const uint64_t cs = d->precision == CYR_FP64 ? 16 : 8, rs = cs / 2;
+  const bool ebasis = d->pipeline == CYR_PIPE_EBASIS;
+  uint64_t b = cs * g.V * (ebasis ? 6 : 3);
+  if (!ebasis && d->algorithm == 1) b += cs * g.V * 6;
+  if (!ebasis && d->scatter_approach == 1) b += rs * g.V;
+  b += rs * g.P * (ebasis ? g.nA : 1) + 8 * g.P * 2 + 4 * g.P;
+  b += 64 * 1024 + (uint64_t)g.nK * g.nA * (16 + 72 + 48) + (uint64_t)g.nE * g.M * 64;
+  return b;
+}
+
+cyr_status run_simulation(const cyr_sim_desc *d, const SimGeometry &g, const void *d_vox, const double *mats,
+                          void *d_out, WorkBuffers &wb, hipStream_t st) {
+  if (d->precision == CYR_FP64)
+    return run_impl<double>(d, g, d_vox, mats, reinterpret_cast<double *>(d_out), wb, st);
+  return run_impl<float>(d, g, d_vox, mats, reinterpret_cast<float *>(d_out), wb, st);
+}
+
+void convert_storage(const cyr_sim_desc *d, const void *host_vox, std::vector<unsigned char> &out) {
+  const uint64_t n = (uint64_t)d->dims[0] * d->dims[1] * d->dims[2] * d->num_materials * 4;
+  out.resize(n * 2);
+  uint16_t *o = reinterpret_cast<uint16_t *>(out.data());
+  for (uint64_t i = 0; i < n; i++) {
+    const float v = d->precision == CYR_FP64 ? (float)reinterpret_cast<const double *>(host_vox)[i]
+                                             : reinterpret_cast<const float *>(host_vox)[i];
+    if (d->storage == CYR_STORE_FP16) {
+      __half h = __float2half(v);
+      std::memcpy(&o[i], &h, 2);
+    } else {
+      // round to nearest even bfloat16
+      uint32_t u;
+      std::memcpy(&u, &v, 4);
+      if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x7fffffu)) o[i] = (uint16_t)((u >> 16) | 0x40);
+      else o[i] = (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+    }
+  }
+}
+
+}  // namespace cyr
+
+using namespace cyr;
+
+extern "C" {
+
+uint64_t cyr_footprint_bytes(const cyr_sim_desc *desc) {
+  SimGeometry g;
+  if (build_geometry(desc, g) != CYR_OK) return 0;
+  return morphology_bytes(desc) + work_bytes(desc, g) + output_bytes(desc, g);
+}
+
+cyr_status cyr_simulate_device(const cyr_sim_desc *desc, const void *d_voxels, const double *mats, void *d_out,
+                               void *stream) {
+  SimGeometry g;
+  CYR_TRY(build_geometry(desc, g));
+  if (!d_voxels || !mats || !d_out) return fail(CYR_E_INVALID, "null buffer");
+  DeviceState *ds;
+  CYR_TRY(device_state(&ds));
+  std::lock_guard<std::mutex> lk(ds->mu);
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  return run_simulation(desc, g, d_voxels, mats, d_out, ds->wb, st);
+}
+
+cyr_status cyr_simulate(const cyr_sim_desc *desc, const void *voxels, const double *mats, void *out) {
+  SimGeometry g;
+  CYR_TRY(build_geometry(desc, g));
+  if (!voxels || !mats || !out) return fail(CYR_E_INVALID, "null buffer");
+  DeviceState *ds;
+  CYR_TRY(device_state(&ds));
+  std::lock_guard<std::mutex> lk(ds->mu);
+  const uint64_t vb = morphology_bytes(desc), ob = output_bytes(desc, g);
+  CYR_TRY(ensure_buffer(ds->vox, vb));
+  CYR_TRY(ensure_buffer(ds->out, ob));
+  if (desc->storage == CYR_STORE_NATIVE) {
+    CYR_HIP(hipMemcpyAsync(ds->vox.ptr, voxels, vb, hipMemcpyHostToDevice, ds->stream));
+  } else {
+    std::vector<unsigned char> conv;
+    convert_storage(desc, voxels, conv);
+    CYR_HIP(hipMemcpyAsync(ds->vox.ptr, conv.data(), vb, hipMemcpyHostToDevice, ds->stream));
+    CYR_HIP(hipStreamSynchronize(ds->stream));
+  }
+  CYR_TRY(run_simulation(desc, g, ds->vox.ptr, mats, ds->out.ptr, ds->wb, ds->stream));
+  CYR_HIP(hipMemcpyAsync(out, ds->out.ptr, ob, hipMemcpyDeviceToHost, ds->stream));
+  CYR_HIP(hipStreamSynchronize(ds->stream));
+  return CYR_OK;
+}
+
+cyr_status cyr_polarization_device(const cyr_sim_desc *desc, const void *d_voxels, const double *mats_e,
+                                   double eangle_rad, void *d_px, void *d_py, void *d_pz, void *stream) {
+  SimGeometry g;
+  CYR_TRY(build_geometry(desc, g));
+  if (desc->case_type != 0)  // src/cudaMain.cu:1677-1680
+    return fail(CYR_E_UNSUPPORTED, "Only implemented for Case Type = 0");
+  if (!d_voxels || !mats_e || !d_px || !d_py || !d_pz) return fail(CYR_E_INVALID, "null buffer");
+  DeviceState *ds;
+  CYR_TRY(device_state(&ds));
+  std::lock_guard<std::mutex> lk(ds->mu);
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  const M3 R = rotation_matrix_e(g.kvec[0], g.RK[0], eangle_rad);
+  const bool hann = desc->windowing == 1;
+  auto go = [&](auto tag) -> cyr_status {
+    using T = decltype(tag);
+    Tables tb;
+    std::vector<double> m(4 * (size_t)g.M * g.nE);
+    for (uint32_t e = 0; e < g.nE; e++) std::memcpy(&m[4 * (size_t)e * g.M], mats_e, sizeof(double) * 4 * g.M);
+    build_tables<T>(desc, g, m.data(), tb);
+    CYR_TRY(ensure_buffer(ds->wb.tables, tb.total));
+    CYR_HIP(hipMemcpyAsync(ds->wb.tables.ptr, tb.host.data(), tb.total, hipMemcpyHostToDevice, st));
+    unsigned char *tbase = reinterpret_cast<unsigned char *>(ds->wb.tables.ptr);
+    PolarArgs<T> pa{};
+    pa.mats = reinterpret_cast<const MatConst<T> *>(tbase + tb.mats);
+    pa.M = g.M;
+    pa.V = g.V;
+    pa.nx = g.nx;
+    pa.ny = g.ny;
+    pa.nz = g.nz;
+    const T *h = reinterpret_cast<const T *>(tbase + tb.han);
+    pa.hx = h;
+    pa.hy = h + g.nx;
+    pa.hz = h + g.nx + g.ny;
+    pa.e[0] = Vec3<T>{(T)R.m[0], (T)R.m[3], (T)R.m[6]};
+    pa.R = m3<T>(R);
+    pa.out[0] = reinterpret_cast<cx<T> *>(d_px);
+    pa.out[1] = reinterpret_cast<cx<T> *>(d_py);
+    pa.out[2] = reinterpret_cast<cx<T> *>(d_pz);
+    CYR_TRY(launch_polarization<T>(pa, d_voxels, desc->storage, desc->morphology_type, 1, hann,
+                                   desc->reference_frame == 0, st));
+    CYR_HIP(hipStreamSynchronize(st));
+    return CYR_OK;
+  };
+  if (desc->precision == CYR_FP64) return go(double{});
+  return go(float{});
+}
+
+cyr_status cyr_polarization(const cyr_sim_desc *desc, const void *voxels, const double *mats, uint32_t energy_id,
+                            double eangle_rad, void *px, void *py, void *pz) {
+  SimGeometry g;
+  CYR_TRY(build_geometry(desc, g));
+  if (energy_id >= g.nE) return fail(CYR_E_INVALID, "energy_id out of range");
+  if (!voxels || !mats || !px || !py || !pz) return fail(CYR_E_INVALID, "null buffer");
+  DeviceState *ds;
+  CYR_TRY(device_state(&ds));
+  const uint64_t vb = morphology_bytes(desc);
+  const uint64_t cb = g.V * (desc->precision == CYR_FP64 ? 16 : 8);
+  DeviceBuffer dv, dp;
+  CYR_TRY(ensure_buffer(dv, vb));
+  cyr_status s = ensure_buffer(dp, 3 * cb);
+  if (s != CYR_OK) {
+    free_buffer(dv);
+    return s;
+  }
+  hipStream_t st = ds->stream;
+  if (desc->storage == CYR_STORE_NATIVE) {
+    s = hipMemcpy(dv.ptr, voxels, vb, hipMemcpyHostToDevice) == hipSuccess ? CYR_OK
+                                                                            : fail(CYR_E_DEVICE, "upload failed");
+  } else {
+    std::vector<unsigned char> conv;
+    convert_storage(desc, voxels, conv);
+    s = hipMemcpy(dv.ptr, conv.data(), vb, hipMemcpyHostToDevice) == hipSuccess ? CYR_OK
+                                                                                : fail(CYR_E_DEVICE, "upload failed");
+  }
+  unsigned char *b = reinterpret_cast<unsigned char *>(dp.ptr);
+  if (s == CYR_OK)
+    s = cyr_polarization_device(desc, dv.ptr, mats + 4 * (size_t)energy_id * g.M, eangle_rad, b, b + cb, b + 2 * cb,
+                                st);
+  if (s == CYR_OK) {
+    if (hipMemcpy(px, b, cb, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(py, b + cb, cb, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(pz, b + 2 * cb, cb, hipMemcpyDeviceToHost) != hipSuccess)
+      s = fail(CYR_E_DEVICE, "download failed");
+  }
+  free_buffer(dv);
+  free_buffer(dp);
+  return s;
+}
+
+cyr_status cyr_fft3d_device(uint32_t nx, uint32_t ny, uint32_t nz, int32_t precision, void *const *d_volumes,
+                            uint32_t count, void *stream) {
+  if (!d_volumes || nx < 1 || ny < 1 || nz < 1) return fail(CYR_E_INVALID, "bad fft arguments");
+  for (uint32_t i = 0; i < count; i++)
+    if (!d_volumes[i]) return fail(CYR_E_INVALID, "null volume");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  if (precision == CYR_FP64) CYR_TRY(fft3d<double>(d_volumes, count, nx, ny, nz, st));
+  else if (precision == CYR_FP32) CYR_TRY(fft3d<float>(d_volumes, count, nx, ny, nz, st));
+  else return fail(CYR_E_INVALID, "precision out of range");
+  CYR_HIP(hipStreamSynchronize(st));
+  return CYR_OK;
+}
+
+cyr_status cyr_dc_shift_device(uint32_t nx, uint32_t ny, uint32_t nz, int32_t precision, void *d_volume,
+                               int32_t do_dc, int32_t do_shift, void *stream) {
+  if (!d_volume || nx < 2 || ny < 2 || nz < 1) return fail(CYR_E_INVALID, "bad dc/shift arguments");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  if (precision == CYR_FP64) {
+    cx<double> *v = reinterpret_cast<cx<double> *>(d_volume);
+    if (do_dc) CYR_TRY(launch_dc_replace<double>(&v, 1, nx, ny, nz, st));
+    if (do_shift) CYR_TRY(launch_igor_shift<double>(v, nx, ny, nz, st));
+  } else if (precision == CYR_FP32) {
+    cx<float> *v = reinterpret_cast<cx<float> *>(d_volume);
+    if (do_dc) CYR_TRY(launch_dc_replace<float>(&v, 1, nx, ny, nz, st));
+    if (do_shift) CYR_TRY(launch_igor_shift<float>(v, nx, ny, nz, st));
+  } else {
+    return fail(CYR_E_INVALID, "precision out of range");
+  }
+  CYR_HIP(hipStreamSynchronize(st));
+  return CYR_OK;
+}
+
+cyr_status cyr_ewald_device(const cyr_sim_desc *desc, uint32_t energy_id, uint32_t k_id, const void *d_px,
+                            const void *d_py, const void *d_pz, void *d_proj, void *stream) {
+  SimGeometry g;
+  CYR_TRY(build_geometry(desc, g));
+  if (energy_id >= g.nE || k_id >= g.nK) return fail(CYR_E_INVALID, "energy_id/k_id out of range");
+  if (!d_px || !d_py || !d_pz || !d_proj) return fail(CYR_E_INVALID, "null buffer");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  const double kv[3] = {g.kvec[k_id].x, g.kvec[k_id].y, g.kvec[k_id].z};
+  const EwaldGeom geo = make_ewald_geom(g.nx, g.ny, g.nz, desc->phys_size, g.kmag[energy_id], kv, desc->interpolation);
+  auto go = [&](auto tag) -> cyr_status {
+    using T = decltype(tag);
+    const cx<T> *px = reinterpret_cast<const cx<T> *>(d_px), *py = reinterpret_cast<const cx<T> *>(d_py),
+                *pz = reinterpret_cast<const cx<T> *>(d_pz);
+    T *proj = reinterpret_cast<T *>(d_proj);
+    if (desc->scatter_approach == 1) {
+      DeviceBuffer sc;
+      CYR_TRY(ensure_buffer(sc, sizeof(T) * g.V));
+      cyr_status s = launch_scatter3d<T>(geo, px, py, pz, reinterpret_cast<T *>(sc.ptr), st);
+      if (s == CYR_OK) s = launch_ewald_full<T>(geo, reinterpret_cast<T *>(sc.ptr), proj, st);
+      if (s == CYR_OK && hipStreamSynchronize(st) != hipSuccess) s = fail(CYR_E_DEVICE, "sync failed");
+      free_buffer(sc);
+      return s;
+    }
+    CYR_TRY(launch_ewald_partial<T>(geo, px, py, pz, proj, st));
+    CYR_HIP(hipStreamSynchronize(st));
+    return CYR_OK;
+  };
+  if (desc->precision == CYR_FP64) return go(double{});
+  return go(float{});
+}
+
+cyr_status cyr_warp_affine_device(uint32_t nx, uint32_t ny, int32_t precision, const void *d_src, void *d_dst,
+                                  const double coeffs[6], void *stream) {
+  if (!d_src || !d_dst || !coeffs || nx < 1 || ny < 1) return fail(CYR_E_INVALID, "bad warp arguments");
+  const double det = coeffs[0] * coeffs[4] - coeffs[1] * coeffs[3];
+  if (!(std::fabs(det) > 0) || !std::isfinite(det)) return fail(CYR_E_INVALID, "singular warp");
+  double inv[6];
+  affine_inverse(coeffs, inv);
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  if (precision == CYR_FP64)
+    CYR_TRY(launch_warp<double>(reinterpret_cast<const double *>(d_src), reinterpret_cast<double *>(d_dst), nx, ny,
+                                inv, st));
+  else if (precision == CYR_FP32)
+    CYR_TRY(launch_warp<float>(reinterpret_cast<const float *>(d_src), reinterpret_cast<float *>(d_dst), nx, ny, inv,
+                               st));
+  else
+    return fail(CYR_E_INVALID, "precision out of range");
+  CYR_HIP(hipStreamSynchronize(st));
+  return CYR_OK;
+}
+
+cyr_status cyr_release_workspace(void) {
+  std::lock_guard<std::mutex> lk(g_states_mu);
+  int cur = 0;
+  hipGetDevice(&cur);
+  for (auto &kv : g_states) {
+    hipSetDevice(kv.first);
+    std::lock_guard<std::mutex> lk2(kv.second->mu);
+    kv.second->wb.release();
+    free_buffer(kv.second->vox);
+    free_buffer(kv.second->out);
+  }
+  hipSetDevice(cur);
+  fft_release();
+  return CYR_OK;
+}
+
+}  // extern "C"

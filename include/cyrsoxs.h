@@ -1,0 +1,194 @@
+/*
+ * cyrsoxs.h — C ABI of the MI355X-native CyRSoXS Born-approximation pipeline.
+ *
+ * Plain pointers, sizes and POD descriptors only; every call returns a
+ * cyr_status and never exits the process (the reference calls exit() on every
+ * device error: include/cudaHeaders.h:43-63).  Citations name the reference
+ * interface each entry point replaces (paths relative to the usnistgov/cyrsoxs
+ * tree).
+ *
+ * Layouts (shared with the reference):
+ *   morphology   [M][Nz][Ny][Nx] x 4 reals per voxel: Euler (S, theta, psi, Vfrac) or
+ *                vector (sx, sy, sz, phi_unaligned)             (include/Input/Input.h:59-94)
+ *   constants    double[nE][M][4] = (npara.re, npara.im, nperp.re, nperp.im),
+ *                n = (1 - delta) + i beta                       (include/Input/InputData.h:371-375)
+ *   patterns     [nE][nK][Ny][Nx] of the compute precision      (src/cudaMain.cu:968-971)
+ *   polarization [Nz][Ny][Nx] complex (re, im) of the compute precision
+ */
+#ifndef CYRSOXS_H
+#define CYRSOXS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  CYR_OK = 0,
+  CYR_E_INVALID = 1,     /* bad descriptor or argument */
+  CYR_E_NOMEM = 2,       /* device or host allocation failed */
+  CYR_E_DEVICE = 3,      /* HIP runtime / kernel launch error */
+  CYR_E_ADMISSION = 4,   /* request larger than the engine's HBM budget */
+  CYR_E_CANCELLED = 5,   /* request cancelled before it ran */
+  CYR_E_UNSUPPORTED = 6, /* option combination the reference does not support either */
+  CYR_E_NODEVICE = 7,    /* no HIP device (reference: src/cudaMain.cu:402-405) */
+  CYR_E_TIMEOUT = 8,     /* wait timed out; the request is still in flight */
+  CYR_E_NOTFOUND = 9     /* unknown request id */
+} cyr_status;
+
+/* Compute precision (the reference's compile-time DOUBLE_PRECISION switch,
+ * include/Datatypes.h:33-46, made a runtime parameter). */
+enum { CYR_FP32 = 0, CYR_FP64 = 1 };
+/* Device storage of the morphology: native = the compute precision. */
+enum { CYR_STORE_NATIVE = 0, CYR_STORE_FP16 = 1, CYR_STORE_BF16 = 2 };
+/* Execution strategy.  PLAIN keeps the reference's per-angle structure
+ * (polarization -> 3 full 3D FFTs -> DC/shift -> Ewald -> rotate/accumulate for
+ * every E angle).  EBASIS uses the linearity of the pipeline in E: two
+ * polarizations per energy, an XY FFT, and only the Ewald-sphere z bins of the
+ * z transform; every angle is then a 2D operation.  Both produce the same
+ * patterns within floating-point rounding. */
+enum { CYR_PIPE_PLAIN = 0, CYR_PIPE_EBASIS = 1 };
+
+/* Mirrors InputData (include/Input/InputData.h:186-232) plus the compile-time
+ * switches of the reference build.  Enumerations use the reference's values
+ * (include/Datatypes.h:56-182). */
+typedef struct {
+  uint32_t dims[3];          /* Nx, Ny, Nz (X fastest); Nz == 1 selects the 2D path */
+  uint32_t num_materials;
+  double phys_size;          /* nm */
+  uint32_t num_energies;
+  const double *energies;    /* eV, length num_energies */
+  double angle_start;        /* E-angle rotation, degrees */
+  double angle_increment;
+  double angle_end;
+  uint32_t num_kvectors;
+  const double *kvectors;    /* [nK][3], normalised; (0,0,1) for case type 0 */
+  double detector[3];        /* normalised detector direction (case type 2) */
+  int32_t case_type;         /* 0 default, 1 beam divergence, 2 grazing incidence */
+  int32_t morphology_type;   /* 0 Euler angles, 1 vector morphology */
+  int32_t reference_frame;   /* 0 MATERIAL, 1 LAB */
+  int32_t interpolation;     /* 0 nearest neighbour, 1 linear */
+  int32_t windowing;         /* 0 none, 1 Hanning */
+  int32_t rot_mask;          /* 0/1 */
+  int32_t scatter_approach;  /* 0 partial, 1 full */
+  int32_t algorithm;         /* 0 communication minimizing, 1 memory minimizing (Nt) */
+  int32_t precision;         /* CYR_FP32 / CYR_FP64 */
+  int32_t storage;           /* CYR_STORE_* (device morphology element type) */
+  int32_t pipeline;          /* CYR_PIPE_* */
+  int32_t reserved;
+} cyr_sim_desc;
+
+/* ---- whole simulation ---------------------------------------------------- */
+
+/* Replaces int cudaMain(...) / int cudaMainStreams(...) (include/cudaMain.h:53-68,
+ * src/cudaMain.cu:377-1667).  voxels: host [M][V][4] of the compute precision;
+ * out: host [nE][nK][Ny][Nx] of the compute precision.  Host buffers are
+ * caller-owned and not retained after return. */
+cyr_status cyr_simulate(const cyr_sim_desc *desc, const void *voxels, const double *mats, void *out);
+
+/* Same, with inputs and outputs already resident in HBM.  d_voxels is
+ * [M][V][4] of the storage type (float/double, or 16-bit for FP16/BF16
+ * storage); d_out is [nE][nK][Ny][Nx] of the compute precision.  Work is
+ * queued on `stream` (a hipStream_t, or NULL for the default stream) and the
+ * call returns after the stream has finished. */
+cyr_status cyr_simulate_device(const cyr_sim_desc *desc, const void *d_voxels, const double *mats,
+                               void *d_out, void *stream);
+
+/* Replaces int computePolarization(...) (include/cudaMain.h:84-87,
+ * src/cudaMain.cu:1669-1738): one polarization for (energy_id, eangle_rad),
+ * k vector 0, copied to three host complex arrays [V] of the compute precision. */
+cyr_status cyr_polarization(const cyr_sim_desc *desc, const void *voxels, const double *mats,
+                            uint32_t energy_id, double eangle_rad, void *px, void *py, void *pz);
+
+/* Device form of the polarization launcher computePolarization(d_materialConstants,
+ * d_voxelInput, ...) (src/cudaMain.cu:299-332): d_p* are [V] complex. */
+cyr_status cyr_polarization_device(const cyr_sim_desc *desc, const void *d_voxels,
+                                   const double *mats_e, double eangle_rad, void *d_px,
+                                   void *d_py, void *d_pz, void *stream);
+
+/* ---- stage-level device entry points (the reference's host launchers) ----- */
+
+/* Forward, unnormalised, in-place 3D C2C FFT of `count` volumes; replaces
+ * performFFT / cufftExecC2C|Z2Z (include/cudaMain.h:184-190). */
+cyr_status cyr_fft3d_device(uint32_t nx, uint32_t ny, uint32_t nz, int32_t precision,
+                            void *const *d_volumes, uint32_t count, void *stream);
+
+/* DC replacement then Igor shift, in place; replaces replaceDCComponent +
+ * performFFTShift (src/cudaMain.cu:57-60,178-181). */
+cyr_status cyr_dc_shift_device(uint32_t nx, uint32_t ny, uint32_t nz, int32_t precision,
+                               void *d_volume, int32_t do_dc, int32_t do_shift, void *stream);
+
+/* Ewald projection for (energy_id, k_id) from UNshifted FFT volumes (after DC
+ * replacement); the Igor shift is applied in the gather addressing.  Replaces
+ * peformEwaldProjectionGPU (src/cudaMain.cu:203-243); d_proj is [Ny][Nx]. */
+cyr_status cyr_ewald_device(const cyr_sim_desc *desc, uint32_t energy_id, uint32_t k_id,
+                            const void *d_px, const void *d_py, const void *d_pz, void *d_proj,
+                            void *stream);
+
+/* Bilinear affine warp of an [Ny][Nx] image with forward coefficients
+ * c = [[c0,c1,c2],[c3,c4,c5]] (source -> destination); destination pixels whose
+ * back-mapped point falls outside the source stay as they were.  Replaces
+ * nppiWarpAffine_{32f,64f}_C1R_Ctx with NPPI_INTER_LINEAR (src/cudaMain.cu:91-121). */
+cyr_status cyr_warp_affine_device(uint32_t nx, uint32_t ny, int32_t precision, const void *d_src,
+                                  void *d_dst, const double coeffs[6], void *stream);
+
+/* ---- batched execution ---------------------------------------------------- */
+
+/* Runs `count` independent simulations (uneven grids allowed) resident in HBM
+ * as one schedule: per-member work is interleaved across streams and the
+ * member's kernel sequence is captured once in a hipGraph per shape.  No
+ * counterpart in the reference (it runs one simulation per call). */
+cyr_status cyr_simulate_batch(uint32_t count, const cyr_sim_desc *descs,
+                              const void *const *d_voxels, const double *const *mats,
+                              void *const *d_outs, void *stream);
+
+/* ---- serving engine ------------------------------------------------------- */
+
+typedef struct cyr_engine_s *cyr_engine;
+
+typedef struct {
+  uint64_t submitted, completed, rejected, cancelled;
+  uint64_t resident_bytes;   /* HBM currently reserved by admitted requests */
+  uint64_t budget_bytes;
+  uint32_t in_flight;
+  uint32_t queued;
+} cyr_engine_stats;
+
+/* Continuous-batching engine on one device: requests are admitted by HBM
+ * footprint (cyr_footprint_bytes) against budget_bytes, each admitted request's
+ * morphology stays resident across its energies, and up to `slots` requests
+ * run concurrently on their own streams.  No counterpart in the reference. */
+cyr_status cyr_engine_create(int device, uint64_t budget_bytes, uint32_t slots, cyr_engine *out);
+/* Copies the host inputs into the engine; `out` (host, [nE][nK][Ny][Nx]) must
+ * stay valid until the request completes. */
+cyr_status cyr_engine_submit(cyr_engine eng, const cyr_sim_desc *desc, const void *voxels,
+                             const double *mats, void *out, uint64_t *request_id);
+cyr_status cyr_engine_wait(cyr_engine eng, uint64_t request_id, double timeout_s,
+                           cyr_status *request_status);
+cyr_status cyr_engine_poll(cyr_engine eng, uint64_t request_id, int *done,
+                           cyr_status *request_status);
+cyr_status cyr_engine_cancel(cyr_engine eng, uint64_t request_id);
+cyr_status cyr_engine_get_stats(cyr_engine eng, cyr_engine_stats *stats);
+cyr_status cyr_engine_destroy(cyr_engine eng);
+
+/* ---- utilities ------------------------------------------------------------- */
+
+const char *cyr_status_string(cyr_status s);
+/* Detail of the last failure on the calling thread ("" if none). */
+const char *cyr_last_error(void);
+int cyr_device_count(void);
+/* numAnglesRotation = round((end - start)/increment + 1) (src/cudaMain.cu:393-394). */
+uint32_t cyr_num_angles(double start, double increment, double end);
+/* Device bytes a simulation holds while it runs (morphology + work volumes). */
+uint64_t cyr_footprint_bytes(const cyr_sim_desc *desc);
+/* Frees the per-device cached work buffers of the plain/E-basis executors. */
+cyr_status cyr_release_workspace(void);
+/* Library version string. */
+const char *cyr_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CYRSOXS_H */
