@@ -1,0 +1,209 @@
+#!/usr/bin/env python3
+"""Throughput benchmark of the MI355X CyRSoXS pipeline.
+
+Workload (default): realistic bulk morphology, 4 materials on a 512x512x128 grid,
+PhysSize 2 nm, Gaussian-filtered softmax volume fractions, filtered S, sphere-
+uniform Euler angles; 20 energies (280-299 eV) x 36 E angles (0:5:175), linear
+Ewald interpolation, Hanning window, FP32.  A step is one whole simulation
+(every energy and angle) with the morphology already resident in HBM.
+
+value = voxel*angle*energy per second of the production (E-basis) pipeline; the
+plain per-angle pipeline (the reference's structure) is timed beside it.  The
+roofline object reports the dominant kernel's achieved HBM bandwidth, timed with
+HIP events on the stream it runs on.  cpu_baseline times the FP64 CPU oracle on a
+bounded sample of the same workload on this host's cores.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+METRIC = "voxel·angle·energy/s + p50/p99 latency, batched vs plain, at ≤1e-4 rel. L2"
+HBM_PEAK_GBS = 8000.0
+
+
+def _pct(xs, p):
+    xs = sorted(xs)
+    if not xs:
+        return None
+    i = max(0, min(len(xs) - 1, int(round(p / 100.0 * len(xs) + 0.5)) - 1))  # nearest rank
+    return xs[i]
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def time_events(torch, fn, reps):
+    """Average duration of fn() over reps launches, measured with HIP events on
+    torch's current stream (the stream the library launches on)."""
+    st = torch.cuda.current_stream()
+    fn()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(reps):
+        fn()
+    e1.record(st)
+    e1.synchronize()
+    return e0.elapsed_time(e1) / reps / 1000.0
+
+
+def cpu_baseline(vox_t, mats, desc, budget_s):
+    """FP64 oracle on 1 energy and as many angles as fit ~budget_s (at least one)."""
+    import numpy as np
+
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from oracle import oracle as O
+
+    log("cpu baseline: copying morphology to host (FP64)")
+    vox = vox_t.double().cpu().numpy()
+    osim = O.Sim(desc["dims"], desc["num_materials"], desc["phys_size"], desc["energies"][:1], desc["angles"],
+                 interpolation=desc["interpolation"], windowing=desc["windowing"])
+    nA = osim.num_angles
+    nx, ny, nz = desc["dims"]
+    V = nx * ny * nz
+    done = 0
+    t_total = 0.0
+    while done < nA:
+        t0 = time.perf_counter()
+        O.simulate(osim, vox, mats[:1], energies=(0, 1), angles=(done, done + 1))
+        t_total += time.perf_counter() - t0
+        done += 1
+        log(f"cpu baseline: {done} angle(s) in {t_total:.1f} s")
+        if t_total > budget_s or t_total / done * (done + 1) > 1.5 * budget_s:
+            break
+    return {"value": V * done / t_total, "unit": "voxel·angle·energy/s", "cores": O.max_threads(),
+            "kind": "port",
+            "sample": f"FP64 oracle, 1 energy x {done} of {nA} angles of the same 512x512x128 4-material workload "
+                      f"({t_total:.1f} s)"}
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--plain-steps", type=int, default=1)
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--kernel-reps", type=int, default=20)
+    a = ap.parse_args(argv)
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus > 1 is not supported (energies and "
+                                   "requests shard across devices without communication; see DESIGN.md)"}))
+        return 2
+    rank = int(os.environ.get("RANK", "0"))
+
+    import numpy as np
+    import torch
+
+    from cyrsoxs_amd import capi, synthetic
+
+    torch.cuda.set_device(0)
+    desc = synthetic.config1_desc()
+    dims, M = desc["dims"], desc["num_materials"]
+    nx, ny, nz = dims
+    V = nx * ny * nz
+    log(f"generating morphology {dims} x {M} materials on the GPU")
+    vox = synthetic.bulk_morphology(dims, M, seed=1, sigma=6.0, device="cuda")
+    mats = synthetic.lorentzian_constants(desc["energies"], M, seed=1)
+    torch.cuda.synchronize()
+    nE = len(desc["energies"])
+
+    def mk(pipeline):
+        return capi.Sim(dims, M, desc["phys_size"], desc["energies"], desc["angles"],
+                        interpolation=desc["interpolation"], windowing=desc["windowing"], precision=capi.FP32,
+                        pipeline=pipeline)
+
+    sim = mk(capi.PIPE_EBASIS)
+    nA = sim.num_angles
+    units = V * nA * nE
+    out = torch.empty((nE, 1, ny, nx), dtype=torch.float32, device="cuda")
+
+    def run(s, o):
+        capi.simulate_device(s, vox, mats, o)
+
+    log(f"E-basis pipeline: warmup {a.warmup}, steps {a.steps}")
+    for _ in range(a.warmup):
+        run(sim, out)
+    torch.cuda.synchronize()
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        s0 = time.perf_counter()
+        run(sim, out)  # returns after the stream has drained
+        lat.append(time.perf_counter() - s0)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    value = units * a.steps / elapsed
+
+    result = {
+        "metric": METRIC, "value": value, "unit": "voxel·angle·energy/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": 1000.0 * elapsed / a.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded Gaussian-filtered bulk morphology, Lorentzian optical constants)",
+        "config": {"workload": "bulk 512x512x128, 4 materials, PhysSize 2 nm, 20 energies x 36 E angles "
+                               "(0:5:175), linear Ewald, Hanning, FP32",
+                   "pipeline": "ebasis (2 polarizations/energy, XY FFT, pruned z DFT at the Ewald bins)",
+                   "units_per_step": units},
+        "latency_ms": {"p50": 1000.0 * _pct(lat, 50), "p99": 1000.0 * _pct(lat, 99)},
+    }
+
+    if not a.no_plain:
+        psim = mk(capi.PIPE_PLAIN)
+        pout = torch.empty_like(out)
+        log(f"plain pipeline: warmup 1, steps {a.plain_steps}")
+        run(psim, pout)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(a.plain_steps):
+            run(psim, pout)
+        torch.cuda.synchronize()
+        pel = time.perf_counter() - t0
+        g = out.double().cpu().numpy()
+        p = pout.double().cpu().numpy()
+        fin = np.isfinite(p)
+        rel = [float(np.linalg.norm((g[e][fin[e]] - p[e][fin[e]])) / np.linalg.norm(p[e][fin[e]]))
+               for e in range(nE)]
+        result["plain"] = {"value": units * a.plain_steps / pel, "ms_per_step": 1000.0 * pel / a.plain_steps,
+                           "steps": a.plain_steps,
+                           "structure": "per angle: polarization, 3 full 3D FFTs, DC, Ewald, rotate/accumulate",
+                           "speedup_ebasis_vs_plain": value / (units * a.plain_steps / pel),
+                           "ebasis_vs_plain_max_rel_l2": max(rel),
+                           "nan_mask_identical": bool(np.array_equal(np.isnan(g), np.isnan(p)))}
+
+    # ---- roofline of the dominant kernels, timed with HIP events
+    cd = torch.complex64
+    vols = [torch.empty((nz, ny, nx), dtype=cd, device="cuda") for _ in range(6)]
+    for v in vols:
+        v.real.normal_()
+        v.imag.normal_()
+    kr = {}
+    for axis, name in ((0, "fft_x"), (1, "fft_y")):
+        t = time_events(torch, lambda: capi.fft_axis_device(vols, dims, capi.FP32, axis), a.kernel_reps)
+        byts = 6 * V * 16  # read + write every complex64 element once
+        kr[name] = {"avg_us": 1e6 * t, "bytes": byts, "GBs": byts / t / 1e9}
+    dom = max(kr, key=lambda k: kr[k]["avg_us"])
+    result["roofline"] = {"bound": "hbm", "kernel": f"fft_pencil_kernel ({dom}, 6 x 512x512x128 complex64)",
+                          "achieved": kr[dom]["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": kr[dom]["GBs"] / HBM_PEAK_GBS, "traffic": None,
+                          "algorithmic_bytes": kr[dom]["bytes"], "avg_us": kr[dom]["avg_us"], "kernels": kr}
+    del vols
+
+    if not a.no_cpu_baseline and rank == 0:
+        result["cpu_baseline"] = cpu_baseline(vox, mats, desc, a.cpu_budget)
+
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

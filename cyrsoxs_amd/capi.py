@@ -78,6 +78,8 @@ _SIGS = {
                                                ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "cyr_fft3d_device": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int32,
                                         ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p]),
+    "cyr_fft_axis_device": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int32,
+                                           ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int32, ctypes.c_void_p]),
     "cyr_dc_shift_device": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int32,
                                            ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p]),
     "cyr_ewald_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p,
@@ -233,6 +235,14 @@ def fft3d_device(vols, dims, precision, stream=None):
     arr = (ctypes.c_void_p * len(vols))(*[v.data_ptr() for v in vols])
     nx, ny, nz = dims
     check(lib().cyr_fft3d_device(nx, ny, nz, precision, arr, len(vols), _stream_ptr(stream)), "cyr_fft3d_device")
+
+
+def fft_axis_device(vols, dims, precision, axis, stream=None):
+    """One axis pass, queued asynchronously on the stream (no synchronisation)."""
+    arr = (ctypes.c_void_p * len(vols))(*[v.data_ptr() for v in vols])
+    nx, ny, nz = dims
+    check(lib().cyr_fft_axis_device(nx, ny, nz, precision, arr, len(vols), axis, _stream_ptr(stream)),
+          "cyr_fft_axis_device")
 
 
 def dc_shift_device(vol, dims, precision, do_dc=True, do_shift=True, stream=None):

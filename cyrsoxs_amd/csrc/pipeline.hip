@@ -478,6 +478,17 @@ cyr_status cyr_fft3d_device(uint32_t nx, uint32_t ny, uint32_t nz, int32_t preci
   return CYR_OK;
 }
 
+cyr_status cyr_fft_axis_device(uint32_t nx, uint32_t ny, uint32_t nz, int32_t precision, void *const *d_volumes,
+                               uint32_t count, int32_t axis, void *stream) {
+  if (!d_volumes || nx < 1 || ny < 1 || nz < 1 || axis < 0 || axis > 2) return fail(CYR_E_INVALID, "bad fft arguments");
+  for (uint32_t i = 0; i < count; i++)
+    if (!d_volumes[i]) return fail(CYR_E_INVALID, "null volume");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  if (precision == CYR_FP64) return fft_axis<double>(d_volumes, count, nx, ny, nz, axis, st);
+  if (precision == CYR_FP32) return fft_axis<float>(d_volumes, count, nx, ny, nz, axis, st);
+  return fail(CYR_E_INVALID, "precision out of range");
+}
+
 cyr_status cyr_dc_shift_device(uint32_t nx, uint32_t ny, uint32_t nz, int32_t precision, void *d_volume,
                                int32_t do_dc, int32_t do_shift, void *stream) {
   if (!d_volume || nx < 2 || ny < 2 || nz < 1) return fail(CYR_E_INVALID, "bad dc/shift arguments");

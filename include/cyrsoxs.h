@@ -115,6 +115,11 @@ cyr_status cyr_polarization_device(const cyr_sim_desc *desc, const void *d_voxel
 cyr_status cyr_fft3d_device(uint32_t nx, uint32_t ny, uint32_t nz, int32_t precision,
                             void *const *d_volumes, uint32_t count, void *stream);
 
+/* One axis (0 = X, 1 = Y, 2 = Z) of the same transform, queued on `stream`
+ * without synchronising (the pass cufftPlan3d performs internally). */
+cyr_status cyr_fft_axis_device(uint32_t nx, uint32_t ny, uint32_t nz, int32_t precision,
+                               void *const *d_volumes, uint32_t count, int32_t axis, void *stream);
+
 /* DC replacement then Igor shift, in place; replaces replaceDCComponent +
  * performFFTShift (src/cudaMain.cu:57-60,178-181). */
 cyr_status cyr_dc_shift_device(uint32_t nx, uint32_t ny, uint32_t nz, int32_t precision,
