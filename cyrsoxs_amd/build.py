@@ -61,8 +61,53 @@ def build_lib(jobs=8):
     return out
 
 
+HOST_SOURCES = ["host/config.cpp", "host/input_data.cpp", "host/h5io.cpp"]
+CXX = os.environ.get("CXX", "g++")
+HOST_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function", "-I", os.path.join(REPO, "include")]
+
+
+def _compile_host(src, extra=()):
+    os.makedirs(OBJDIR, exist_ok=True)
+    obj = os.path.join(OBJDIR, src.replace("/", "_") + ".o")
+    _run([CXX, *HOST_FLAGS, *extra, "-c", os.path.join(CSRC, src), "-o", obj])
+    return obj
+
+
+def build_cli(jobs=8):
+    """The CyRSoXS executable (reference src/main.cpp) next to the library."""
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        objs = list(ex.map(_compile_host, HOST_SOURCES + ["host/main.cpp"]))
+    out = os.path.join(LIBDIR, "CyRSoXS")
+    _run([CXX, "-o", out + ".tmp", *objs, "-L", LIBDIR, "-lcyrsoxs", "-Wl,-rpath,$ORIGIN", "-ldl"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def build_pymodule(jobs=8):
+    """The pybind11 module CyRSoXS (reference src/pymain.cpp) in the package directory."""
+    try:
+        import pybind11
+    except ImportError:
+        return None
+    pyinc = sysconfig.get_paths()["include"]
+    if not os.path.exists(os.path.join(pyinc, "Python.h")):
+        return None
+    extra = ["-I", pybind11.get_include(), "-I", pyinc, "-fvisibility=hidden"]
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        objs = list(ex.map(_compile_host, HOST_SOURCES))
+        objs.append(ex.submit(_compile_host, "host/pymain.cpp", extra).result())
+    out = os.path.join(HERE, "CyRSoXS" + sysconfig.get_config_var("EXT_SUFFIX"))
+    _run([CXX, "-shared", "-o", out + ".tmp", *objs, "-L", LIBDIR, "-lcyrsoxs", "-Wl,-rpath,$ORIGIN/lib", "-ldl"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_all(jobs=8, only=None):
     lib = build_lib(jobs)
+    if only == "lib":
+        return lib
+    build_cli(jobs)
+    build_pymodule(jobs)
     return lib
 
 
