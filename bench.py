@@ -9,8 +9,10 @@ Ewald interpolation, Hanning window, FP32.  A step is one whole simulation
 
 value = voxel*angle*energy per second of the production (E-basis) pipeline; the
 plain per-angle pipeline (the reference's structure) is timed beside it.  The
-roofline object reports the dominant kernel's achieved HBM bandwidth, timed with
-HIP events on the stream it runs on.  cpu_baseline times the FP64 CPU oracle on a
+roofline object reports the dominant kernel's achieved HBM bandwidth: every
+launch of the timed steps is bracketed by HIP events on the library stream it
+runs on (cyr_kernel_timing), and its algorithmic bytes (inputs read once,
+outputs written once) are divided by its average duration.  cpu_baseline times the FP64 CPU oracle on a
 bounded sample of the same workload on this host's cores.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
@@ -25,6 +27,10 @@ import time
 
 METRIC = "voxel·angle·energy/s + p50/p99 latency, batched vs plain, at ≤1e-4 rel. L2"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this
+# workload, scaled by the counters' reading for a copy kernel of known size
+# (profiles/ holds the runs); None where not measured.
+TRAFFIC = {}
 
 
 def _pct(xs, p):
@@ -37,22 +43,6 @@ def _pct(xs, p):
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
-
-
-def time_events(torch, fn, reps):
-    """Average duration of fn() over reps launches, measured with HIP events on
-    torch's current stream (the stream the library launches on)."""
-    st = torch.cuda.current_stream()
-    fn()
-    torch.cuda.synchronize()
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    e0.record(st)
-    for _ in range(reps):
-        fn()
-    e1.record(st)
-    e1.synchronize()
-    return e0.elapsed_time(e1) / reps / 1000.0
 
 
 def cpu_baseline(vox_t, mats, desc, budget_s):
@@ -94,7 +84,6 @@ def main(argv=None):
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
-    ap.add_argument("--kernel-reps", type=int, default=20)
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus > 1 is not supported (energies and "
@@ -135,6 +124,9 @@ def main(argv=None):
     for _ in range(a.warmup):
         run(sim, out)
     torch.cuda.synchronize()
+    # every launch of the timed steps is bracketed by HIP events on the stream it
+    # is queued on (the library's stream); the per-kernel averages give the roofline
+    capi.kernel_timing(True)
     lat = []
     t0 = time.perf_counter()
     for _ in range(a.steps):
@@ -143,7 +135,10 @@ def main(argv=None):
         lat.append(time.perf_counter() - s0)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    capi.kernel_timing(False)
+    kern = capi.kernel_timing_report()
     value = units * a.steps / elapsed
+    log(f"E-basis: {1000.0 * elapsed / a.steps:.2f} ms/step")
 
     result = {
         "metric": METRIC, "value": value, "unit": "voxel·angle·energy/s", "n_gpus": 1,
@@ -180,23 +175,19 @@ def main(argv=None):
                            "ebasis_vs_plain_max_rel_l2": max(rel),
                            "nan_mask_identical": bool(np.array_equal(np.isnan(g), np.isnan(p)))}
 
-    # ---- roofline of the dominant kernels, timed with HIP events
-    cd = torch.complex64
-    vols = [torch.empty((nz, ny, nx), dtype=cd, device="cuda") for _ in range(6)]
-    for v in vols:
-        v.real.normal_()
-        v.imag.normal_()
-    kr = {}
-    for axis, name in ((0, "fft_x"), (1, "fft_y")):
-        t = time_events(torch, lambda: capi.fft_axis_device(vols, dims, capi.FP32, axis), a.kernel_reps)
-        byts = 6 * V * 16  # read + write every complex64 element once
-        kr[name] = {"avg_us": 1e6 * t, "bytes": byts, "GBs": byts / t / 1e9}
-    dom = max(kr, key=lambda k: kr[k]["avg_us"])
-    result["roofline"] = {"bound": "hbm", "kernel": f"fft_pencil_kernel ({dom}, 6 x 512x512x128 complex64)",
-                          "achieved": kr[dom]["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                          "frac": kr[dom]["GBs"] / HBM_PEAK_GBS, "traffic": None,
-                          "algorithmic_bytes": kr[dom]["bytes"], "avg_us": kr[dom]["avg_us"], "kernels": kr}
-    del vols
+    # ---- roofline of the dominant kernel (largest share of the timed region)
+    tot = sum(k["total_us"] for k in kern.values())
+    table = {}
+    for name, k in sorted(kern.items(), key=lambda kv: -kv[1]["total_us"]):
+        gbs = k["avg_bytes"] / (k["avg_us"] * 1e-6) / 1e9 if k["avg_us"] > 0 else 0.0
+        table[name] = {"calls": k["calls"], "avg_us": round(k["avg_us"], 2), "share": round(k["total_us"] / tot, 4),
+                       "algorithmic_bytes": k["avg_bytes"], "GBs": round(gbs, 1)}
+    dom = max(kern, key=lambda n: kern[n]["total_us"])
+    ach = table[dom]["GBs"]
+    result["roofline"] = {"bound": "hbm", "kernel": f"{dom}_kernel", "achieved": ach, "peak": HBM_PEAK_GBS,
+                          "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": TRAFFIC.get(dom),
+                          "algorithmic_bytes": kern[dom]["avg_bytes"], "avg_us": kern[dom]["avg_us"],
+                          "kernels": table}
 
     if not a.no_cpu_baseline and rank == 0:
         result["cpu_baseline"] = cpu_baseline(vox, mats, desc, a.cpu_budget)

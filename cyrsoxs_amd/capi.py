@@ -103,6 +103,8 @@ _SIGS = {
     "cyr_footprint_bytes": (ctypes.c_uint64, [ctypes.c_void_p]),
     "cyr_release_workspace": (ctypes.c_int, []),
     "cyr_version": (ctypes.c_char_p, []),
+    "cyr_kernel_timing": (ctypes.c_int, [ctypes.c_int]),
+    "cyr_kernel_timing_report": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t]),
 }
 
 
@@ -273,6 +275,22 @@ def simulate_batch(sims, d_voxels, mats, d_outs, stream=None):
     mp = (ctypes.c_void_p * n)(*[m.ctypes.data for m in mats_c])
     op = (ctypes.c_void_p * n)(*[o.data_ptr() for o in d_outs])
     check(lib().cyr_simulate_batch(n, descs, vp, mp, op, _stream_ptr(stream)), "cyr_simulate_batch")
+
+
+def kernel_timing(enable):
+    check(lib().cyr_kernel_timing(1 if enable else 0), "cyr_kernel_timing")
+
+
+def kernel_timing_report():
+    """{kernel: {"calls", "total_us", "avg_us", "avg_bytes"}} accumulated since
+    kernel_timing(True); avg_bytes is the launch's algorithmic HBM traffic."""
+    buf = ctypes.create_string_buffer(1 << 16)
+    lib().cyr_kernel_timing_report(buf, len(buf))
+    out = {}
+    for line in buf.value.decode().splitlines():
+        name, calls, tot, avg, byts = line.split()
+        out[name] = {"calls": int(calls), "total_us": float(tot), "avg_us": float(avg), "avg_bytes": float(byts)}
+    return out
 
 
 def device_count():

@@ -121,4 +121,15 @@ struct DeviceBuffer {
 cyr_status ensure_buffer(DeviceBuffer &b, size_t bytes);
 void free_buffer(DeviceBuffer &b);
 
+// ---------------------------------------------------------------------------
+// Opt-in per-kernel event timing (cyr_kernel_timing).  kt_begin/kt_end bracket
+// one launch on `st`; `bytes` is the launch's algorithmic HBM traffic (each
+// input read once, each output written once).  kt_collect (after the stream has
+// drained) folds the recorded pairs into the per-name totals.
+// ---------------------------------------------------------------------------
+bool kt_enabled();
+void kt_begin(hipStream_t st);
+void kt_end(const char *name, hipStream_t st, double bytes = 0.0);
+void kt_collect();
+
 }  // namespace cyr

@@ -144,156 +144,6 @@ __global__ void __launch_bounds__(256) igor_shift_kernel(cx<T> *vol, uint32_t nx
   }
 }
 
-// DC replacement values of the six XY-transformed volumes G(kx, ky, z): the
-// neighbours of bin (0,0,0) are sums over z of G at kz = 0 (in-plane
-// neighbours) or kz = 1, Nz-1 (z neighbours).  One block per volume.
-template <typename T> struct Six { cx<T> *v[6]; };
-
-template <typename T>
-__global__ void __launch_bounds__(256) ebasis_dc_kernel(Six<T> g6, uint32_t nx, uint32_t ny, uint32_t nz,
-                                                        const cx<T> *__restrict__ twz, cx<T> *dcval) {
-  __shared__ T red[256][12];
-  const cx<T> *G = g6.v[blockIdx.x];
-  const uint64_t P = (uint64_t)nx * ny;
-  const uint64_t cols[4] = {1, nx, (uint64_t)nx - 1, (uint64_t)(ny - 1) * nx};
-  T acc[12];
-#pragma unroll
-  for (int i = 0; i < 12; i++) acc[i] = 0;
-  for (uint32_t z = threadIdx.x; z < nz; z += blockDim.x) {
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-      const cx<T> v = G[(uint64_t)z * P + cols[i]];
-      acc[2 * i] += v.x;
-      acc[2 * i + 1] += v.y;
-    }
-    if (nz > 1) {
-      const cx<T> v = G[(uint64_t)z * P];
-      const cx<T> w1 = twz[z % nz];                                    // kz = 1
-      const cx<T> w2 = twz[(uint32_t)(((uint64_t)z * (nz - 1)) % nz)];  // kz = nz - 1
-      const cx<T> a = cmul<T>(v, w1), b = cmul<T>(v, w2);
-      acc[8] += a.x;
-      acc[9] += a.y;
-      acc[10] += b.x;
-      acc[11] += b.y;
-    }
-  }
-#pragma unroll
-  for (int i = 0; i < 12; i++) red[threadIdx.x][i] = acc[i];
-  __syncthreads();
-  for (uint32_t s = blockDim.x / 2; s > 0; s >>= 1) {
-    if (threadIdx.x < s) {
-#pragma unroll
-      for (int i = 0; i < 12; i++) red[threadIdx.x][i] += red[threadIdx.x + s][i];
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    const int cnt = nz > 1 ? 6 : 4;
-    T sr = 0, si = 0;
-    for (int i = 0; i < cnt; i++) {
-      sr += red[0][2 * i];
-      si += red[0][2 * i + 1];
-    }
-    dcval[blockIdx.x] = mk<T>(sr / (T)cnt, si / (T)cnt);
-  }
-}
-
-// E-basis Ewald: per detector pixel, the z DFT of the six XY-transformed
-// volumes is evaluated only at the pixel's Ewald bin(s), then every angle's
-// intensity follows from P(theta) = c P1 + s P2 (x R^T in the MATERIAL frame).
-template <typename T, bool LINEAR3D, bool THREED, bool MATFRAME>
-__global__ void __launch_bounds__(256) ebasis_ewald_kernel(EbasisArgs<T> a) {
-  const EwaldGeom &g = a.g;
-  const uint64_t P = (uint64_t)g.nx * g.ny;
-  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= P) return;
-  const uint32_t Y = (uint32_t)(t / g.nx), X = (uint32_t)(t - (uint64_t)Y * g.nx);
-  uint32_t z0 = 0;
-  double f = 0;
-  const int kind = ewald_geometry(g, X, Y, &z0, &f);
-  if (kind == 0) {
-    for (uint32_t an = 0; an < a.nA; an++) a.proj[(uint64_t)an * P + t] = nan_t<T>();
-    return;
-  }
-  const uint32_t ix = unshift(X, g.nx), iy = unshift(Y, g.ny);
-  const uint64_t col = (uint64_t)iy * g.nx + ix;
-  constexpr int NG = LINEAR3D ? 2 : 1;
-  uint32_t kz[NG];
-  kz[0] = THREED ? unshift(z0, g.nz) : 0;
-  if constexpr (LINEAR3D) kz[1] = unshift(z0 + 1, g.nz);
-  cx<T> acc[NG][6];
-#pragma unroll
-  for (int q = 0; q < NG; q++)
-#pragma unroll
-    for (int c = 0; c < 6; c++) acc[q][c] = mk<T>(0, 0);
-  if constexpr (THREED) {
-    uint32_t e[NG];
-#pragma unroll
-    for (int q = 0; q < NG; q++) e[q] = 0;
-    for (uint32_t z = 0; z < g.nz; z++) {
-      cx<T> w[NG];
-#pragma unroll
-      for (int q = 0; q < NG; q++) w[q] = a.twz[e[q]];
-#pragma unroll
-      for (int c = 0; c < 6; c++) {
-        const cx<T> v = a.vol[c][(uint64_t)z * P + col];
-#pragma unroll
-        for (int q = 0; q < NG; q++) {
-          acc[q][c].x += v.x * w[q].x - v.y * w[q].y;
-          acc[q][c].y += v.x * w[q].y + v.y * w[q].x;
-        }
-      }
-#pragma unroll
-      for (int q = 0; q < NG; q++) {
-        e[q] += kz[q];
-        if (e[q] >= g.nz) e[q] -= g.nz;
-      }
-    }
-  } else {
-#pragma unroll
-    for (int c = 0; c < 6; c++) acc[0][c] = a.vol[c][col];
-  }
-  if (ix == 0 && iy == 0) {
-#pragma unroll
-    for (int q = 0; q < NG; q++)
-      if (kz[q] == 0)
-#pragma unroll
-        for (int c = 0; c < 6; c++) acc[q][c] = a.dcval[c];
-  }
-  double md[NG][6];
-#pragma unroll
-  for (int q = 0; q < NG; q++) scatter_matrix(g, X, Y, THREED ? z0 + q : 0, md[q]);
-  const T w1 = (T)f;
-  for (uint32_t an = 0; an < a.nA; an++) {
-    const T cc = a.coef[2 * an], ss = a.coef[2 * an + 1];
-    T I[NG];
-#pragma unroll
-    for (int q = 0; q < NG; q++) {
-      cx<T> p[3];
-#pragma unroll
-      for (int c = 0; c < 3; c++)
-        p[c] = mk<T>(cc * acc[q][c].x + ss * acc[q][c + 3].x, cc * acc[q][c].y + ss * acc[q][c + 3].y);
-      if constexpr (MATFRAME) {
-        const T *R = a.rmat + 9 * (uint64_t)an;
-        cx<T> r[3];
-#pragma unroll
-        for (int i = 0; i < 3; i++) {
-          r[i].x = R[0 + i] * p[0].x + R[3 + i] * p[1].x + R[6 + i] * p[2].x;
-          r[i].y = R[0 + i] * p[0].y + R[3 + i] * p[1].y + R[6 + i] * p[2].y;
-        }
-        p[0] = r[0];
-        p[1] = r[1];
-        p[2] = r[2];
-      }
-      I[q] = scatter_intensity<T>(md[q], p);
-    }
-    T out;
-    if constexpr (LINEAR3D) out = (1 - w1) * I[0] + w1 * I[1];
-    else out = I[0];
-    a.proj[(uint64_t)an * P + t] = out;
-  }
-}
-
 }  // namespace
 
 template <typename T>
@@ -341,33 +191,6 @@ template <typename T> cyr_status launch_igor_shift(cx<T> *vol, uint32_t nx, uint
   return CYR_OK;
 }
 
-template <typename T>
-cyr_status launch_ebasis_dc(cx<T> *const *g6, uint32_t nx, uint32_t ny, uint32_t nz, const cx<T> *twz, cx<T> *dcval,
-                            hipStream_t st) {
-  Six<T> s{};
-  for (int i = 0; i < 6; i++) s.v[i] = g6[i];
-  hipLaunchKernelGGL((ebasis_dc_kernel<T>), dim3(6), dim3(256), 0, st, s, nx, ny, nz, twz, dcval);
-  CYR_LAUNCHED();
-  return CYR_OK;
-}
-
-template <typename T> cyr_status launch_ebasis_ewald(const EbasisArgs<T> &a, bool matframe, hipStream_t st) {
-  const uint64_t P = (uint64_t)a.g.nx * a.g.ny;
-  const dim3 grid(ceil_div(P, 256)), block(256);
-  const bool three = !a.g.enable2d;
-  const bool lin = three && a.g.interpolation == 1;
-#define CYR_EB(L, TH, MF) \
-  if (lin == L && three == TH && matframe == MF) { hipLaunchKernelGGL((ebasis_ewald_kernel<T, L, TH, MF>), grid, block, 0, st, a); CYR_LAUNCHED(); return CYR_OK; }
-  CYR_EB(true, true, false)
-  CYR_EB(true, true, true)
-  CYR_EB(false, true, false)
-  CYR_EB(false, true, true)
-  CYR_EB(false, false, false)
-  CYR_EB(false, false, true)
-#undef CYR_EB
-  return fail(CYR_E_INVALID, "ebasis ewald: bad mode");
-}
-
 #define CYR_INST(T)                                                                                                  \
   template cyr_status launch_ewald_partial<T>(const EwaldGeom &, const cx<T> *, const cx<T> *, const cx<T> *, T *,   \
                                               hipStream_t);                                                          \
@@ -375,10 +198,7 @@ template <typename T> cyr_status launch_ebasis_ewald(const EbasisArgs<T> &a, boo
                                           hipStream_t);                                                              \
   template cyr_status launch_ewald_full<T>(const EwaldGeom &, const T *, T *, hipStream_t);                          \
   template cyr_status launch_dc_replace<T>(cx<T> *const *, uint32_t, uint32_t, uint32_t, uint32_t, hipStream_t);     \
-  template cyr_status launch_igor_shift<T>(cx<T> *, uint32_t, uint32_t, uint32_t, hipStream_t);                      \
-  template cyr_status launch_ebasis_dc<T>(cx<T> *const *, uint32_t, uint32_t, uint32_t, const cx<T> *, cx<T> *,      \
-                                          hipStream_t);                                                              \
-  template cyr_status launch_ebasis_ewald<T>(const EbasisArgs<T> &, bool, hipStream_t);
+  template cyr_status launch_igor_shift<T>(cx<T> *, uint32_t, uint32_t, uint32_t, hipStream_t);
 CYR_INST(float)
 CYR_INST(double)
 #undef CYR_INST

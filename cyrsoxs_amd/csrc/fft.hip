@@ -24,20 +24,11 @@
 #include <vector>
 
 #include "fft.hpp"
+#include "stockham.hpp"
 
 namespace cyr {
 
 namespace {
-
-constexpr int kMaxStages = 16;
-
-struct StagePlan {
-  uint32_t n;
-  uint32_t nstages;
-  uint32_t radix[kMaxStages];
-  uint32_t ls[kMaxStages];
-  uint32_t tstep[kMaxStages];
-};
 
 struct PassArgs {
   void *vol[8];
@@ -50,81 +41,6 @@ struct PassArgs {
   uint32_t pp;
   uint32_t tpp;
 };
-
-// ---- radix butterflies (forward, W = exp(-2 pi i / R)) ----------------------
-
-template <typename T> __device__ inline void bfly2(cx<T> *a) {
-  cx<T> t = a[0];
-  a[0] = cadd<T>(t, a[1]);
-  a[1] = csub<T>(t, a[1]);
-}
-
-template <typename T> __device__ inline void bfly4(cx<T> *a) {
-  cx<T> t0 = cadd<T>(a[0], a[2]), t1 = csub<T>(a[0], a[2]);
-  cx<T> t2 = cadd<T>(a[1], a[3]), t3 = cmul_mi<T>(csub<T>(a[1], a[3]));
-  a[0] = cadd<T>(t0, t2);
-  a[1] = cadd<T>(t1, t3);
-  a[2] = csub<T>(t0, t2);
-  a[3] = csub<T>(t1, t3);
-}
-
-template <typename T> __device__ inline void bfly8(cx<T> *a) {
-  const T r = (T)0.70710678118654752440084436210484903928;
-  cx<T> e[4] = {a[0], a[2], a[4], a[6]};
-  cx<T> o[4] = {a[1], a[3], a[5], a[7]};
-  bfly4<T>(e);
-  bfly4<T>(o);
-  // W8^1 = (1 - i)/sqrt2, W8^2 = -i, W8^3 = -(1 + i)/sqrt2
-  cx<T> o1 = mk<T>((o[1].x + o[1].y) * r, (o[1].y - o[1].x) * r);
-  cx<T> o2 = cmul_mi<T>(o[2]);
-  cx<T> o3 = mk<T>((o[3].y - o[3].x) * r, -(o[3].x + o[3].y) * r);
-  a[0] = cadd<T>(e[0], o[0]);
-  a[4] = csub<T>(e[0], o[0]);
-  a[1] = cadd<T>(e[1], o1);
-  a[5] = csub<T>(e[1], o1);
-  a[2] = cadd<T>(e[2], o2);
-  a[6] = csub<T>(e[2], o2);
-  a[3] = cadd<T>(e[3], o3);
-  a[7] = csub<T>(e[3], o3);
-}
-
-template <typename T> __device__ inline void bfly3(cx<T> *a) {
-  const T s = (T)0.86602540378443864676372317075293618347;  // sin(2 pi/3)
-  cx<T> t1 = cadd<T>(a[1], a[2]);
-  cx<T> t2 = csub<T>(a[1], a[2]);
-  cx<T> m = mk<T>(a[0].x - (T)0.5 * t1.x, a[0].y - (T)0.5 * t1.y);
-  cx<T> u = mk<T>(s * t2.y, -s * t2.x);  // -i s t2
-  a[0] = cadd<T>(a[0], t1);
-  a[1] = cadd<T>(m, u);
-  a[2] = csub<T>(m, u);
-}
-
-template <typename T> __device__ inline void bfly5(cx<T> *a) {
-  const T c1 = (T)0.30901699437494742410229341718281905886;   // cos(2 pi/5)
-  const T c2 = (T)-0.80901699437494742410229341718281905886;  // cos(4 pi/5)
-  const T s1 = (T)0.95105651629515357211643933337938214340;   // sin(2 pi/5)
-  const T s2 = (T)0.58778525229247312916870595463907276860;   // sin(4 pi/5)
-  cx<T> b1 = cadd<T>(a[1], a[4]), b2 = cadd<T>(a[2], a[3]);
-  cx<T> d1 = csub<T>(a[1], a[4]), d2 = csub<T>(a[2], a[3]);
-  cx<T> m1 = mk<T>(a[0].x + c1 * b1.x + c2 * b2.x, a[0].y + c1 * b1.y + c2 * b2.y);
-  cx<T> m2 = mk<T>(a[0].x + c2 * b1.x + c1 * b2.x, a[0].y + c2 * b1.y + c1 * b2.y);
-  cx<T> n1 = mk<T>(s1 * d1.x + s2 * d2.x, s1 * d1.y + s2 * d2.y);
-  cx<T> n2 = mk<T>(s2 * d1.x - s1 * d2.x, s2 * d1.y - s1 * d2.y);
-  a[0] = mk<T>(a[0].x + b1.x + b2.x, a[0].y + b1.y + b2.y);
-  // y = m -/+ i n
-  a[1] = mk<T>(m1.x + n1.y, m1.y - n1.x);
-  a[4] = mk<T>(m1.x - n1.y, m1.y + n1.x);
-  a[2] = mk<T>(m2.x + n2.y, m2.y - n2.x);
-  a[3] = mk<T>(m2.x - n2.y, m2.y + n2.x);
-}
-
-template <typename T, int R> __device__ inline void bfly(cx<T> *a) {
-  if constexpr (R == 2) bfly2<T>(a);
-  else if constexpr (R == 3) bfly3<T>(a);
-  else if constexpr (R == 4) bfly4<T>(a);
-  else if constexpr (R == 5) bfly5<T>(a);
-  else bfly8<T>(a);
-}
 
 // ---- one Stockham stage ------------------------------------------------------
 //
@@ -334,6 +250,25 @@ size_t max_lds_bytes() {
 }
 
 }  // namespace
+
+bool make_stage_plan(uint32_t n, StagePlan &P, uint32_t &lcm) {
+  const Factoring f = factor(n);
+  if (!f.ok) return false;
+  P = StagePlan{};
+  P.n = n;
+  P.nstages = (uint32_t)f.radices.size();
+  uint32_t Ls = 1;
+  for (uint32_t s = 0; s < P.nstages; s++) {
+    P.radix[s] = f.radices[s];
+    P.ls[s] = Ls;
+    P.tstep[s] = n / (Ls * f.radices[s]);
+    Ls *= f.radices[s];
+  }
+  lcm = f.lcm;
+  return true;
+}
+
+size_t device_max_lds() { return max_lds_bytes(); }
 
 template <typename T> const cx<T> *twiddles(uint32_t n, cyr_status *st) {
   int dev = 0;

@@ -5,6 +5,7 @@
 #include <cstdint>
 
 #include "common.hpp"
+#include "stockham.hpp"
 
 namespace cyr {
 
@@ -30,5 +31,10 @@ void fft_set_shape(uint32_t n, int contig, int precision, FftShape s);
 void fft_clear_shapes();
 
 void fft_release();
+
+// Stockham stage plan of an axis length made of factors 2, 3, 5 (false otherwise);
+// lcm = least common multiple of the radices (the minimum elements per thread).
+bool make_stage_plan(uint32_t n, StagePlan &P, uint32_t &lcm);
+size_t device_max_lds();
 
 }  // namespace cyr

@@ -623,7 +623,15 @@ PYBIND11_MODULE(CyRSoXS, module) {
       .def("validate", &PyVoxelData::validate, "validate the Voxel data")
       .def("readFromH5", &PyVoxelData::readFromH5, "Reads from HDF5", py::arg("Filename"))
       .def("reset", &PyVoxelData::reset, "Resets the voxel data")
-      .def("writeToH5", &PyVoxelData::writeToH5, "Writes voxel data to HDF5 file");
+      .def("writeToH5", &PyVoxelData::writeToH5, "Writes voxel data to HDF5 file")
+      .def(
+          "dataToNumpy",
+          [](const PyVoxelData &v) {
+            py::array_t<double> a({(py::ssize_t)v.in->d.NUM_MATERIAL, (py::ssize_t)v.V(), (py::ssize_t)4});
+            std::memcpy(a.mutable_data(), v.vox.data(), v.vox.size() * sizeof(double));
+            return a;
+          },
+          "Copy of the morphology as [NumMaterial][Nz*Ny*Nx][4] (X fastest)");
 
   py::class_<PyScatteringPattern>(module, "ScatteringPattern")
       .def(py::init<PyInputData &>(), "Constructor", py::arg("InputData"), py::keep_alive<1, 2>())

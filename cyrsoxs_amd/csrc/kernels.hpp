@@ -59,22 +59,62 @@ cyr_status launch_dc_replace(cx<T> *const *vols, uint32_t count, uint32_t nx, ui
 template <typename T>
 cyr_status launch_igor_shift(cx<T> *vol, uint32_t nx, uint32_t ny, uint32_t nz, hipStream_t st);
 
-// E-basis path: DC replacement values of the six XY-transformed volumes.
-template <typename T>
-cyr_status launch_ebasis_dc(cx<T> *const *g6, uint32_t nx, uint32_t ny, uint32_t nz, const cx<T> *twz,
-                            cx<T> *dcval, hipStream_t st);
-// E-basis path: pruned z DFT at the Ewald bins + every angle's projection.
-template <typename T> struct EbasisArgs {
-  EwaldGeom g;
-  const cx<T> *vol[6];   // XY-transformed volumes (basis 1 xyz, basis 2 xyz)
-  const cx<T> *twz;      // z twiddles (length nz) or null in 2D
-  const cx<T> *dcval;    // [6] DC replacement values
-  const T *coef;         // [nA][2] E = c e1 + s e2
-  const T *rmat;         // [nA][9] E rotations (MATERIAL frame) or null
-  uint32_t nA;
-  T *proj;               // [nA][Ny][Nx]
+// ---- E-basis executor (ebasis.hip) -----------------------------------------
+// True when both in-plane axis lengths have a Stockham plan that fits the fused kernels.
+bool ebasis_supported(uint32_t nx, uint32_t ny, int precision);
+// Grid size the Y/z kernel's z chunking aims for, and the most chunks any plan
+// for (nx, nz) uses (sizes the partial-sum buffer).
+constexpr uint32_t kYzTargetBlocks = 1024;
+uint32_t ebasis_max_chunks(uint32_t nx, uint32_t nz);
+// Pixel -> FFT z bin of its (kx, ky) column: kzt[q][ky][kx] (-1 = not sampled).
+cyr_status launch_ebasis_bins(const EwaldGeom &g, int ng, int32_t *kzt, hipStream_t st);
+
+template <typename T> struct PolxArgs {
+  const MatConst<T> *mats;
+  uint32_t M;
+  uint64_t V;
+  uint32_t nx, ny, nz, rows;       // rows = ny * nz
+  const T *hx, *hy, *hz;           // Hanning tables
+  Vec3<T> e[2];                    // E basis
+  const cx<T> *twx;                // filled by the launcher
+  uint32_t tpp, pp;                // filled by the launcher
+  cx<T> *out[6];                   // X-transformed volumes (basis 1 xyz, basis 2 xyz)
 };
-template <typename T> cyr_status launch_ebasis_ewald(const EbasisArgs<T> &a, bool matframe, hipStream_t st);
+template <typename T>
+cyr_status launch_ebasis_polx(PolxArgs<T> a, const void *vox, int storage, int morph, bool hann, hipStream_t st);
+
+template <typename T> struct YzArgs {
+  const cx<T> *vol[6];             // X-transformed volumes
+  cx<T> *part;                     // [nchunks][NG][6][Ny][Nx] partial sums
+  const int32_t *kzt;              // [NG][Ny][Nx]
+  const cx<T> *twy, *twz;          // twy filled by the launcher; twz null in 2D
+  uint32_t nx, ny, nz;
+  uint32_t e, pp, tpp, zchunk, nchunks;  // 0 = automatic (launcher fills)
+};
+// Fills e, pp, tpp, zchunk, nchunks (fields left 0 are chosen automatically;
+// target_blocks is the grid size the z chunking aims for).
+template <typename T> cyr_status ebasis_yz_plan(YzArgs<T> &a, uint32_t target_blocks);
+template <typename T> cyr_status launch_ebasis_yz(YzArgs<T> a, int ng, hipStream_t st);
+
+template <typename T> struct DcxArgs {
+  const cx<T> *vol[6];
+  const cx<T> *twy, *twz;
+  uint32_t nx, ny, nz;
+  cx<T> *dcval;                    // [6]
+};
+template <typename T> cyr_status launch_ebasis_dcx(const DcxArgs<T> &a, hipStream_t st);
+
+template <typename T> struct AnglesArgs {
+  EwaldGeom g;
+  const cx<T> *part;
+  uint32_t nchunks;
+  const cx<T> *dcval;
+  const T *coef;                   // [nA][2]
+  const T *rmat;                   // [nA][9] (MATERIAL frame)
+  uint32_t nA;
+  T *proj;                         // [nA][Ny][Nx]
+};
+template <typename T> cyr_status launch_ebasis_angles(const AnglesArgs<T> &a, int ng, bool matframe, hipStream_t st);
 
 // ---- rotation / averaging (rotate.hip) --------------------------------------
 // Plain path: rotate one projection by the E angle and accumulate (FP64).

@@ -119,7 +119,9 @@ template <typename T>
 cyr_status run_impl(const cyr_sim_desc *d, const SimGeometry &g, const void *d_vox, const double *mats, T *d_out,
                     WorkBuffers &wb, hipStream_t st) {
   const uint64_t V = g.V, P = g.P;
-  const bool ebasis = d->pipeline == CYR_PIPE_EBASIS;
+  // the fused E-basis kernels need 2^a 3^b 5^c in-plane lengths; others run the plain executor
+  const bool ebasis = d->pipeline == CYR_PIPE_EBASIS && ebasis_supported(g.nx, g.ny, d->precision) &&
+                      !(d->precision == CYR_FP64 && d->storage != CYR_STORE_NATIVE);
   const bool alg1 = d->algorithm == 1;
   const bool hann = d->windowing == 1 && !alg1;  // Algorithm 1 has no window (include/uniaxial.h:441-521)
   const bool matframe = d->reference_frame == 0;
@@ -160,6 +162,28 @@ cyr_status run_impl(const cyr_sim_desc *d, const SimGeometry &g, const void *d_v
   cyr_status s;
   const cx<T> *twz = g.nz > 1 ? twiddles<T>(g.nz, &s) : nullptr;
   if (g.nz > 1 && !twz) return s;
+  const cx<T> *twy = twiddles<T>(g.ny, &s);
+  if (!twy) return s;
+  // E-basis: pixel -> z-bin table and the Y/z plan (sizes the partial-sum buffer)
+  const int ng = (g.nz > 1 && d->interpolation == 1) ? 2 : 1;
+  int32_t *kzt = nullptr;
+  cx<T> *part = nullptr;
+  YzArgs<T> yz{};
+  if (ebasis) {
+    yz.nx = g.nx;
+    yz.ny = g.ny;
+    yz.nz = g.nz;
+    CYR_TRY(ebasis_yz_plan<T>(yz, kYzTargetBlocks));
+    CYR_TRY(ensure_buffer(wb.bins, sizeof(int32_t) * P * ng));
+    CYR_TRY(ensure_buffer(wb.part, sizeof(cx<T>) * P * ng * 6 * yz.nchunks));
+    kzt = reinterpret_cast<int32_t *>(wb.bins.ptr);
+    part = reinterpret_cast<cx<T> *>(wb.part.ptr);
+  }
+  // algorithmic HBM bytes of the timed launches (cyr_kernel_timing)
+  const double cxb = (double)sizeof(cx<T>);
+  const double vox_el = d->storage == CYR_STORE_NATIVE ? (double)sizeof(T) : 2.0;
+  const double vox_bytes = 4.0 * vox_el * g.M * V;
+  const double part_bytes = cxb * P * ng * 6 * yz.nchunks;
 
   PolarArgs<T> pa{};
   pa.M = g.M;
@@ -181,23 +205,61 @@ cyr_status run_impl(const cyr_sim_desc *d, const SimGeometry &g, const void *d_v
       const EwaldGeom geo = make_ewald_geom(g.nx, g.ny, g.nz, d->phys_size, g.kmag[e], kv, d->interpolation);
       T *out = d_out + ((uint64_t)e * g.nK + k) * P;
       if (ebasis) {
-        pa.e[0] = v3<T>(g.e1[k]);
-        pa.e[1] = v3<T>(g.e2[k]);
-        CYR_TRY(launch_polarization<T>(pa, d_vox, d->storage, d->morphology_type, 2, hann, false, st));
-        CYR_TRY(fft_axis<T>(vptr, 6, g.nx, g.ny, g.nz, 0, st));
-        CYR_TRY(fft_axis<T>(vptr, 6, g.nx, g.ny, g.nz, 1, st));
-        CYR_TRY(launch_ebasis_dc<T>(vols, g.nx, g.ny, g.nz, twz, dcval, st));
-        EbasisArgs<T> ea{};
-        ea.g = geo;
-        for (int c = 0; c < 6; c++) ea.vol[c] = vols[c];
-        ea.twz = twz;
-        ea.dcval = dcval;
-        ea.coef = d_coef + (size_t)2 * k * g.nA;
-        ea.rmat = d_rmat + (size_t)9 * k * g.nA;
-        ea.nA = g.nA;
-        ea.proj = proj;
-        CYR_TRY(launch_ebasis_ewald<T>(ea, matframe, st));
+        // bins -> polarization + X FFT -> DC values -> Y FFT + z bins -> angles -> rotation
+        kt_begin(st);
+        CYR_TRY(launch_ebasis_bins(geo, ng, kzt, st));
+        kt_end("ebasis_bins", st, (double)P * ng * 4);
+        PolxArgs<T> xa{};
+        xa.mats = pa.mats;
+        xa.M = g.M;
+        xa.V = V;
+        xa.nx = g.nx;
+        xa.ny = g.ny;
+        xa.nz = g.nz;
+        xa.rows = g.ny * g.nz;
+        xa.hx = pa.hx;
+        xa.hy = pa.hy;
+        xa.hz = pa.hz;
+        xa.e[0] = v3<T>(g.e1[k]);
+        xa.e[1] = v3<T>(g.e2[k]);
+        for (int c = 0; c < 6; c++) xa.out[c] = vols[c];
+        kt_begin(st);
+        CYR_TRY(launch_ebasis_polx<T>(xa, d_vox, d->storage, d->morphology_type, hann, st));
+        kt_end("ebasis_polx", st, vox_bytes + 6.0 * V * cxb);
+        DcxArgs<T> da{};
+        for (int c = 0; c < 6; c++) da.vol[c] = vols[c];
+        da.twy = twy;
+        da.twz = twz;
+        da.nx = g.nx;
+        da.ny = g.ny;
+        da.nz = g.nz;
+        da.dcval = dcval;
+        kt_begin(st);
+        CYR_TRY(launch_ebasis_dcx<T>(da, st));
+        kt_end("ebasis_dcx", st);
+        YzArgs<T> ya = yz;
+        for (int c = 0; c < 6; c++) ya.vol[c] = vols[c];
+        ya.part = part;
+        ya.kzt = kzt;
+        ya.twz = twz;
+        kt_begin(st);
+        CYR_TRY(launch_ebasis_yz<T>(ya, ng, st));
+        kt_end("ebasis_yz", st, 6.0 * V * cxb + (double)ng * P * 4 + part_bytes);
+        AnglesArgs<T> aa{};
+        aa.g = geo;
+        aa.part = part;
+        aa.nchunks = yz.nchunks;
+        aa.dcval = dcval;
+        aa.coef = d_coef + (size_t)2 * k * g.nA;
+        aa.rmat = d_rmat + (size_t)9 * k * g.nA;
+        aa.nA = g.nA;
+        aa.proj = proj;
+        kt_begin(st);
+        CYR_TRY(launch_ebasis_angles<T>(aa, ng, matframe, st));
+        kt_end("ebasis_angles", st, part_bytes + (double)g.nA * P * sizeof(T));
+        kt_begin(st);
         CYR_TRY(launch_rotate_all<T>(proj, d_rotinv + (size_t)6 * k * g.nA, g.nA, g.nx, g.ny, d->rot_mask, avg, st));
+        kt_end("rotate_all", st, (double)(g.nA + 1) * P * sizeof(T));
       } else {
         CYR_TRY(fill<double>(acc, 0.0, P, st));
         CYR_TRY(fill<uint32_t>(count, 0u, P, st));
@@ -205,21 +267,31 @@ cyr_status run_impl(const cyr_sim_desc *d, const SimGeometry &g, const void *d_v
           const M3 &R = g.R[(size_t)k * g.nA + a];
           pa.e[0] = Vec3<T>{(T)R.m[0], (T)R.m[3], (T)R.m[6]};  // E = R x^
           pa.R = m3<T>(R);
+          kt_begin(st);
           if (alg1)
             CYR_TRY(launch_polarization_nt<T>(pa, reinterpret_cast<const cx<T> *>(wb.nt.ptr), 1, matframe, st));
           else
             CYR_TRY(launch_polarization<T>(pa, d_vox, d->storage, d->morphology_type, 1, hann, matframe, st));
+          kt_end(alg1 ? "polarization_nt" : "polarization", st, (alg1 ? 6.0 * V * cxb : vox_bytes) + 3.0 * V * cxb);
+          kt_begin(st);
           CYR_TRY(fft3d<T>(vptr, 3, g.nx, g.ny, g.nz, st));
+          kt_end("fft3d_3vol", st, (g.nz > 1 ? 3.0 : 2.0) * 3 * 2 * V * cxb);
+          kt_begin(st);
           CYR_TRY(launch_dc_replace<T>(vols, 3, g.nx, g.ny, g.nz, st));
+          kt_end("dc_replace", st);
           if (full) {
             T *sc = reinterpret_cast<T *>(wb.scatter.ptr);
             CYR_TRY(launch_scatter3d<T>(geo, vols[0], vols[1], vols[2], sc, st));
             CYR_TRY(launch_ewald_full<T>(geo, sc, proj, st));
           } else {
+            kt_begin(st);
             CYR_TRY(launch_ewald_partial<T>(geo, vols[0], vols[1], vols[2], proj, st));
+            kt_end("ewald_partial", st, 3.0 * ng * P * cxb + (double)P * sizeof(T));
           }
+          kt_begin(st);
           CYR_TRY(launch_rotate_accumulate<T>(proj, acc, count, g.nx, g.ny, &g.rot_inv[((size_t)k * g.nA + a) * 6],
                                               d->rot_mask, st));
+          kt_end("rotate_accumulate", st, (double)P * (sizeof(T) + 2 * (8 + 4)));
         }
         CYR_TRY(launch_average(acc, count, avg, P, g.nA, d->rot_mask, st));
       }
@@ -228,6 +300,7 @@ cyr_status run_impl(const cyr_sim_desc *d, const SimGeometry &g, const void *d_v
   }
   // keep the host table alive until the upload has been consumed
   CYR_HIP(hipStreamSynchronize(st));
+  kt_collect();
   return CYR_OK;
 }
 
@@ -273,11 +346,13 @@ void WorkBuffers::release() {
   free_buffer(avg);
   free_buffer(tables);
   free_buffer(dc);
+  free_buffer(bins);
+  free_buffer(part);
 }
 
 size_t WorkBuffers::bytes() const {
   return pol.bytes + nt.bytes + scatter.bytes + proj.bytes + acc.bytes + count.bytes + avg.bytes + tables.bytes +
-         dc.bytes;
+         dc.bytes + bins.bytes + part.bytes;
 }
 
 uint64_t morphology_bytes(const cyr_sim_desc *d) {
@@ -292,8 +367,13 @@ uint64_t output_bytes(const cyr_sim_desc *d, const SimGeometry &g) {
 
 uint64_t work_bytes(const cyr_sim_desc *d, const SimGeometry &g) {
   const uint64_t cs = d->precision == CYR_FP64 ? 16 : 8, rs = cs / 2;
-  const bool ebasis = d->pipeline == CYR_PIPE_EBASIS;
+  // the fused E-basis kernels need 2^a 3^b 5^c in-plane lengths; others run the plain executor
+  const bool ebasis = d->pipeline == CYR_PIPE_EBASIS && ebasis_supported(g.nx, g.ny, d->precision) &&
+                      !(d->precision == CYR_FP64 && d->storage != CYR_STORE_NATIVE);
   uint64_t b = cs * g.V * (ebasis ? 6 : 3);
+  if (ebasis) {  // pixel -> bin table and the Y/z partial sums (chunks x 2 bins x 6 volumes)
+    b += 4 * g.P * 2 + cs * g.P * 2 * 6 * ebasis_max_chunks(g.nx, g.nz);
+  }
   if (!ebasis && d->algorithm == 1) b += cs * g.V * 6;
   if (!ebasis && d->scatter_approach == 1) b += rs * g.V;
   b += rs * g.P * (ebasis ? g.nA : 1) + 8 * g.P * 2 + 4 * g.P;

@@ -8,7 +8,7 @@
 namespace cyr {
 
 struct WorkBuffers {
-  DeviceBuffer pol, nt, scatter, proj, acc, count, avg, tables, dc;
+  DeviceBuffer pol, nt, scatter, proj, acc, count, avg, tables, dc, bins, part;
   void release();
   size_t bytes() const;
 };

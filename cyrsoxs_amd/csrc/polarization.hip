@@ -20,87 +20,16 @@
 
 #include "common.hpp"
 #include "kernels.hpp"
+#include "tensor.hpp"
 
 namespace cyr {
 
 namespace {
 
-__device__ inline void sincos_t(float x, float *s, float *c) {
-  *s = __sinf(x);
-  *c = __cosf(x);
-}
-__device__ inline void sincos_t(double x, double *s, double *c) { sincos(x, s, c); }
-
-template <typename T, typename S, int MORPH>
-__device__ inline void accumulate_tensor(const S *__restrict__ vox, const MatConst<T> *__restrict__ mats,
-                                         uint32_t M, uint64_t V, uint64_t idx, T nr[6], T ni[6]) {
-#pragma unroll
-  for (int j = 0; j < 6; j++) nr[j] = ni[j] = 0;
-  for (uint32_t m = 0; m < M; m++) {
-    const MatConst<T> mc = mats[m];
-    T v[4];
-    VoxLoad<T, S>::get(vox, (uint64_t)m * V + idx, v);
-    T sx, sy, sz, pa, pu;
-    if (MORPH == 1) {
-      sx = v[0];
-      sy = v[1];
-      sz = v[2];
-      pu = v[3];
-      pa = (T)1;
-    } else {
-      T st, ct, sp, cp;
-      sincos_t(v[1], &st, &ct);
-      sincos_t(v[2], &sp, &cp);
-      sx = cp * st;
-      sy = sp * st;
-      sz = ct;
-      pa = v[3] * v[0];
-      pu = v[3] - pa;
-    }
-    const T xx = sx * sx, yy = sy * sy, zz = sz * sz;
-    const T xy = pa * sx * sy, xz = pa * sx * sz, yz = pa * sy * sz;
-    // diagonal
-    nr[0] += pa * (mc.ar * xx + mc.br * (yy + zz)) + pu * mc.cr;
-    ni[0] += pa * (mc.ai * xx + mc.bi * (yy + zz)) + pu * mc.ci;
-    nr[3] += pa * (mc.ar * yy + mc.br * (xx + zz)) + pu * mc.cr;
-    ni[3] += pa * (mc.ai * yy + mc.bi * (xx + zz)) + pu * mc.ci;
-    nr[5] += pa * (mc.ar * zz + mc.br * (xx + yy)) + pu * mc.cr;
-    ni[5] += pa * (mc.ai * zz + mc.bi * (xx + yy)) + pu * mc.ci;
-    // off-diagonal
-    nr[1] += mc.dr * xy;
-    ni[1] += mc.di * xy;
-    nr[2] += mc.dr * xz;
-    ni[2] += mc.di * xz;
-    nr[4] += mc.dr * yz;
-    ni[4] += mc.di * yz;
-  }
-}
-
-// p = N e / 4pi (N symmetric: [0 1 2; 1 3 4; 2 4 5])
-template <typename T>
-__device__ inline void apply_tensor(const T nr[6], const T ni[6], T ex, T ey, T ez, cx<T> p[3]) {
-  const T k = (T)(1.0 / (4.0 * M_PI));
-  p[0] = mk<T>((nr[0] * ex + nr[1] * ey + nr[2] * ez) * k, (ni[0] * ex + ni[1] * ey + ni[2] * ez) * k);
-  p[1] = mk<T>((nr[1] * ex + nr[3] * ey + nr[4] * ez) * k, (ni[1] * ex + ni[3] * ey + ni[4] * ez) * k);
-  p[2] = mk<T>((nr[2] * ex + nr[4] * ey + nr[5] * ez) * k, (ni[2] * ex + ni[4] * ey + ni[5] * ez) * k);
-}
-
-// p <- R^T p  (rotate<true>, include/Rotation.h:376-384)
-template <typename T> __device__ inline void rotate_t(const Mat3<T> &R, cx<T> p[3]) {
-  cx<T> q[3];
-#pragma unroll
-  for (int i = 0; i < 3; i++) {
-    q[i].x = R.m[0 + i] * p[0].x + R.m[3 + i] * p[1].x + R.m[6 + i] * p[2].x;
-    q[i].y = R.m[0 + i] * p[0].y + R.m[3 + i] * p[1].y + R.m[6 + i] * p[2].y;
-  }
-  p[0] = q[0];
-  p[1] = q[1];
-  p[2] = q[2];
-}
-
 // Block = a run of `blockDim.x` voxels inside one X row (scalar row decode).
-template <typename T, typename S, int MORPH, int NB, bool HANN, bool MATFRAME>
-__global__ void __launch_bounds__(256) polar_kernel(PolarArgs<T> a, const S *__restrict__ vox) {
+template <typename T, typename S, int MORPH, int NB>
+__global__ void __launch_bounds__(256) polar_kernel(PolarArgs<T> a, const S *__restrict__ vox, int hann,
+                                                    int matframe) {
   const uint32_t row = blockIdx.x / a.blocks_per_row;
   const uint32_t X = (blockIdx.x - row * a.blocks_per_row) * blockDim.x + threadIdx.x;
   if (X >= a.nx) return;
@@ -108,7 +37,7 @@ __global__ void __launch_bounds__(256) polar_kernel(PolarArgs<T> a, const S *__r
   T nr[6], ni[6];
   accumulate_tensor<T, S, MORPH>(vox, a.mats, a.M, a.V, idx, nr, ni);
   T w = (T)1;
-  if (HANN) {
+  if (hann) {
     const uint32_t Y = row % a.ny, Z = row / a.ny;
     w = a.hx[X] * a.hy[Y] * a.hz[Z];
   }
@@ -116,12 +45,9 @@ __global__ void __launch_bounds__(256) polar_kernel(PolarArgs<T> a, const S *__r
   for (int b = 0; b < NB; b++) {
     cx<T> p[3];
     apply_tensor<T>(nr, ni, a.e[b].x, a.e[b].y, a.e[b].z, p);
-    if (MATFRAME) rotate_t<T>(a.R, p);
+    if (matframe) rotate_t<T>(a.R, p);
 #pragma unroll
-    for (int c = 0; c < 3; c++) {
-      if (HANN) p[c] = cscale<T>(p[c], w);
-      a.out[b * 3 + c][idx] = p[c];
-    }
+    for (int c = 0; c < 3; c++) a.out[b * 3 + c][idx] = cscale<T>(p[c], w);
   }
 }
 
@@ -158,36 +84,19 @@ __global__ void __launch_bounds__(256) polar_nt_kernel(PolarArgs<T> a, const cx<
   }
 }
 
-template <typename T, typename S, int MORPH, int NB, bool HANN>
-hipError_t launch_polar_s(const PolarArgs<T> &a, const void *vox, bool matframe, hipStream_t st) {
-  dim3 grid(a.blocks_per_row * a.rows), block(a.block);
-  if (matframe)
-    hipLaunchKernelGGL((polar_kernel<T, S, MORPH, NB, HANN, true>), grid, block, 0, st, a, (const S *)vox);
-  else
-    hipLaunchKernelGGL((polar_kernel<T, S, MORPH, NB, HANN, false>), grid, block, 0, st, a, (const S *)vox);
-  return hipGetLastError();
-}
-
 template <typename T, typename S>
 hipError_t launch_polar_dispatch(const PolarArgs<T> &a, const void *vox, int morph, int nb, bool hann,
                                  bool matframe, hipStream_t st) {
-#define CYR_P(MO, NBV, H) \
-  if (morph == MO && nb == NBV && hann == H) return launch_polar_s<T, S, MO, NBV, H>(a, vox, matframe, st);
-  CYR_P(0, 1, false)
-  CYR_P(0, 1, true)
-  CYR_P(0, 2, false)
-  CYR_P(0, 2, true)
-  CYR_P(1, 1, false)
-  CYR_P(1, 1, true)
-  CYR_P(1, 2, false)
-  CYR_P(1, 2, true)
-#undef CYR_P
-  return hipErrorInvalidValue;
+  const dim3 grid(a.blocks_per_row * a.rows), block(a.block);
+  const int h = hann ? 1 : 0, mf = matframe ? 1 : 0;
+  const S *v = (const S *)vox;
+  if (morph == 0 && nb == 1) hipLaunchKernelGGL((polar_kernel<T, S, 0, 1>), grid, block, 0, st, a, v, h, mf);
+  else if (morph == 0 && nb == 2) hipLaunchKernelGGL((polar_kernel<T, S, 0, 2>), grid, block, 0, st, a, v, h, mf);
+  else if (morph == 1 && nb == 1) hipLaunchKernelGGL((polar_kernel<T, S, 1, 1>), grid, block, 0, st, a, v, h, mf);
+  else if (morph == 1 && nb == 2) hipLaunchKernelGGL((polar_kernel<T, S, 1, 2>), grid, block, 0, st, a, v, h, mf);
+  else return hipErrorInvalidValue;
+  return hipGetLastError();
 }
-
-template <typename T> struct NativeVox;
-template <> struct NativeVox<float> { using type = float4; };
-template <> struct NativeVox<double> { using type = double4; };
 
 }  // namespace
 

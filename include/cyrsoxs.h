@@ -193,6 +193,17 @@ cyr_status cyr_release_workspace(void);
 /* Library version string. */
 const char *cyr_version(void);
 
+/* Opt-in per-kernel timing: while enabled, the executors bracket every launch
+ * with HIP events on the launch stream and accumulate the durations per kernel.
+ * cyr_kernel_timing(1) also clears the accumulated figures.  No counterpart in
+ * the reference (its PROFILING build times stages with host clocks,
+ * src/cudaMain.cu:407-434). */
+cyr_status cyr_kernel_timing(int enable);
+/* Writes "name calls total_us avg_us avg_bytes" lines into buf (NUL-terminated,
+ * truncated to len), avg_bytes being the launch's algorithmic HBM traffic (each
+ * input read once, each output written once); returns the number of kernels. */
+int cyr_kernel_timing_report(char *buf, size_t len);
+
 #ifdef __cplusplus
 }
 #endif

@@ -121,3 +121,21 @@ def test_repeatable_bitwise():
         a = capi.simulate(sim, c["voxels"], c["mats"])
         b = capi.simulate(sim, c["voxels"], c["mats"])
         np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("storage,tol", [(capi.STORE_FP16, 2e-2), (capi.STORE_BF16, 1e-1)])
+def test_half_precision_morphology_storage(storage, tol):
+    """FP16/BF16 morphology storage (FP32 compute): masks identical, error reported."""
+    dims, M = (32, 32, 16), 3
+    vox = random_euler(dims, M, 31)
+    energies = [284.0, 286.0]
+    mats = random_constants(2, M, 5)
+    kw = dict(interpolation=1, windowing=1)
+    ref = _oracle("half_storage", O.Sim(dims, M, 5.0, energies, (0, 30, 150), **kw), vox, mats)
+    for pipeline in (capi.PIPE_PLAIN, capi.PIPE_EBASIS):
+        sim = capi.Sim(dims, M, 5.0, energies, (0, 30, 150), precision=capi.FP32, storage=storage, pipeline=pipeline,
+                       **kw)
+        got = capi.simulate(sim, vox, mats)
+        err, same = pattern_rel_l2(got, ref)
+        print(f"storage={storage} pipeline={pipeline}: rel L2 {err:.3e}")
+        assert same and err < tol
