@@ -1,30 +1,42 @@
 // Fused kernels of the E-basis executor.
 //
-// Per (energy, k vector) the E-basis pipeline needs, for every detector pixel,
-// the FFT of the two E-basis polarizations p1 = N e1 / 4pi, p2 = N e2 / 4pi
-// (6 complex volumes) only at the pixel's Ewald bin(s).  Four launches:
+// The polarization is p(theta) = N E(theta) / 4pi with N the per-voxel tensor
+// sum over materials and E(theta) = c(theta) e1 + s(theta) e2 in the plane
+// normal to k.  Every later stage up to the intensity is linear (Hanning, FFT,
+// DC replacement) or a permutation (Igor shift), so per (energy, k) the
+// pipeline transforms the tensor field once and forms every angle's p~ from it.
+// In the frame Q = [e1 e2 k^] the tensor is N' = Q^T N Q and
+//     Q^T p = N' (c, s, 0)^T / 4pi,
+// which needs only five fields of N': F0 = N'00, F1 = N'01, F2 = N'02,
+// F3 = N'11, F4 = N'12.  For a uniaxial material (include/uniaxial.h:106-150,
+// 215-240) the tensor is
+//     N = sum_m [ iso_m I + d_m phi_a,m s_m s_m^T ],   d = n_par^2 - n_perp^2,
+//     iso = phi_a (n_perp^2 - 1) + phi_u (n_iso^2 - 1)      (Euler, |s| = 1)
+//     iso = (n_perp^2 - 1) |s|^2 + phi_u (n_iso^2 - 1)      (vector, phi_a = 1)
+// (the reference's "- Vfrac" on the diagonal is folded in analytically).
 //
-//   ebasis_bins_kernel   pixel -> FFT z bin(s) of its (kx, ky) column (FP64 geometry)
-//   ebasis_polx_kernel   polarization evaluated directly at the X-FFT's Stockham
-//                        stage-0 positions, X FFT in LDS, X-transformed volumes Gx
-//                        written once (the polarization never reaches HBM)
-//   ebasis_yz_kernel     per z slice: Y FFT of a group of x columns in LDS, each
-//                        output (kx, ky) accumulated into its column's z bin(s) with
-//                        the z twiddle: F(kx,ky,kz) = sum_z Gy(kx,ky,z) W^(z kz).
-//                        Blocks split z into chunks and write per-chunk partial sums
-//                        (deterministic; no atomics).  Gx is read exactly once.
+// Four launches per (energy, k):
+//   ebasis_bins_kernel   pixel -> FFT z bin of its (kx, ky) column (FP64 geometry);
+//                        with linear interpolation the second bin is bin - 1 (mod Nz)
+//   ebasis_polx_kernel   the five fields evaluated at the X FFT's Stockham stage-0
+//                        positions, X FFT in LDS, X-transformed fields Gx written once
+//   ebasis_yz_kernel     per z slice: Y FFT of a group of x columns, each output
+//                        (kx, ky) accumulated into its column's z bin(s):
+//                        F(kx,ky,kz) = sum_z Gy(kx,ky,z) W^(z kz); z is split into
+//                        chunks with per-chunk partial sums (deterministic, no
+//                        atomics).  The DC replacement's neighbour bins are
+//                        accumulated on the side.  Gx is read exactly once.
 //   ebasis_angles_kernel sums the chunks, applies the DC replacement, and for every
-//                        angle forms P = c P1 + s P2 (x R^T in the MATERIAL frame)
-//                        and the Ewald intensity (linear in z when requested).
+//                        angle forms p~ = A(theta) F (A = frame x E coefficients,
+//                        3 x 5, FP64-built) and the Ewald intensity.
 //
-// HBM traffic per (energy, k): morphology V (16 M) + Gx write 48 V + Gx read 48 V,
-// versus (16 M + 48) V + 4 x 48 V + 48 V for polarization, X FFT, Y FFT and the
-// pruned z DFT as separate passes.
+// HBM traffic per (energy, k): morphology V (16 M) + Gx write 40 V + Gx read 40 V.
 #include <hip/hip_runtime.h>
 
 #include "fft.hpp"
 #include "geom.hpp"
 #include "kernels.hpp"
+#include "fftpolicy.hpp"
 #include "stockham.hpp"
 #include "tensor.hpp"
 
@@ -36,8 +48,8 @@ template <typename T> __device__ inline T qnan();
 template <> __device__ inline float qnan<float>() { return __int_as_float(0x7fc00000); }
 template <> __device__ inline double qnan<double>() { return __longlong_as_double(0x7ff8000000000000ll); }
 
-// ---- pixel -> FFT z bin(s) -----------------------------------------------------
-__global__ void __launch_bounds__(256) ebasis_bins_kernel(EwaldGeom g, int ng, int32_t *__restrict__ kzt) {
+// ---- pixel -> FFT z bin ------------------------------------------------------
+__global__ void __launch_bounds__(256) ebasis_bins_kernel(EwaldGeom g, int32_t *__restrict__ kzt) {
   const uint64_t P = (uint64_t)g.nx * g.ny;
   const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= P) return;
@@ -46,172 +58,257 @@ __global__ void __launch_bounds__(256) ebasis_bins_kernel(EwaldGeom g, int ng, i
   double f = 0;
   const int kind = ewald_geometry(g, X, Y, &z0, &f);
   const uint64_t col = (uint64_t)unshift(Y, g.ny) * g.nx + unshift(X, g.nx);
-  const bool three = !g.enable2d;
-  kzt[col] = kind == 0 ? -1 : (int32_t)(three ? unshift(z0, g.nz) : 0);
-  if (ng == 2) kzt[P + col] = kind == 2 ? (int32_t)unshift(z0 + 1, g.nz) : -1;
+  kzt[col] = kind == 0 ? -1 : (int32_t)(g.enable2d ? 0 : unshift(z0, g.nz));
 }
 
-// ---- polarization + X FFT -------------------------------------------------------
-template <typename T, typename S, int E>
-__global__ void __launch_bounds__(256) ebasis_polx_kernel(StagePlan P, PolxArgs<T> a, const S *__restrict__ vox,
-                                                          int morph, int hann) {
+// ---- the five tensor fields of one voxel -------------------------------------
+template <typename T, typename S, int MORPH>
+__device__ inline void tensor_fields(const S *__restrict__ vox, const MatConst<T> *__restrict__ mats, uint32_t M,
+                                     uint64_t V, uint64_t idx, const Mat3<T> &Q, bool rot, T fr[kEbFields],
+                                     T fi[kEbFields]) {
+#pragma unroll
+  for (int j = 0; j < kEbFields; j++) fr[j] = fi[j] = 0;
+  for (uint32_t m = 0; m < M; m++) {
+    const MatConst<T> mc = mats[m];
+    T v[4];
+    VoxLoad<T, S>::get(vox, (uint64_t)m * V + idx, v);
+    T sx, sy, sz, pa, isor, isoi;
+    if (MORPH == 1) {
+      sx = v[0];
+      sy = v[1];
+      sz = v[2];
+      pa = (T)1;
+      const T ss = sx * sx + sy * sy + sz * sz;
+      isor = mc.br * ss + mc.cr * v[3];
+      isoi = mc.bi * ss + mc.ci * v[3];
+    } else {
+      T st, ct, sp, cp;
+      sincos_t(v[1], &st, &ct);
+      sincos_t(v[2], &sp, &cp);
+      sx = cp * st;
+      sy = sp * st;
+      sz = ct;
+      pa = v[3] * v[0];
+      const T pu = v[3] - pa;
+      isor = mc.br * pa + mc.cr * pu;
+      isoi = mc.bi * pa + mc.ci * pu;
+    }
+    if (rot) {  // s' = Q^T s
+      const T a = Q.m[0] * sx + Q.m[3] * sy + Q.m[6] * sz;
+      const T b = Q.m[1] * sx + Q.m[4] * sy + Q.m[7] * sz;
+      const T c = Q.m[2] * sx + Q.m[5] * sy + Q.m[8] * sz;
+      sx = a;
+      sy = b;
+      sz = c;
+    }
+    const T tx = pa * sx, ty = pa * sy;
+    const T q00 = tx * sx, q01 = tx * sy, q02 = tx * sz, q11 = ty * sy, q12 = ty * sz;
+    fr[0] += isor + mc.dr * q00;
+    fi[0] += isoi + mc.di * q00;
+    fr[1] += mc.dr * q01;
+    fi[1] += mc.di * q01;
+    fr[2] += mc.dr * q02;
+    fi[2] += mc.di * q02;
+    fr[3] += isor + mc.dr * q11;
+    fi[3] += isoi + mc.di * q11;
+    fr[4] += mc.dr * q12;
+    fi[4] += mc.di * q12;
+  }
+}
+
+// ---- tensor fields + X FFT ------------------------------------------------------
+// A block holds pp rows of tpp threads (F::kE elements each); WG = 64 makes
+// every row exchange wave-local (the barriers compile to LDS waits).
+template <typename T, typename S, class F, int WG>
+__global__ void __launch_bounds__(WG) ebasis_polx_kernel(StagePlan P, PolxArgs<T> a, const S *__restrict__ vox,
+                                                         int morph, int hann) {
+  constexpr int E = F::kE;
   extern __shared__ unsigned char smem_raw[];
   cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
+  const F fft(P, a.tpp, a.nx, true);
   const uint32_t tpp = a.tpp;
   const uint32_t t = threadIdx.x % tpp, p = threadIdx.x / tpp;
   const uint32_t row = blockIdx.x * a.pp + p;  // row = y + ny z
   const bool active = row < a.rows;
   const uint32_t y = active ? row % a.ny : 0, z = active ? row / a.ny : 0;
-  const uint32_t R0 = P.radix[0], m0 = P.n / R0;
-  const T wyz = hann ? a.hy[y] * a.hz[z] : (T)1;
-  cx<T> vall[6][E];
+  const T k4 = (T)(1.0 / (4.0 * M_PI));
+  const T wyz = hann ? a.hy[y] * a.hz[z] * k4 : k4;
+  const bool rot = a.rot != 0;
+  T fre[kEbFields][E], fim[kEbFields][E];
 #pragma unroll
   for (int i = 0; i < E; i++) {
-    const uint32_t x = t + ((uint32_t)i / R0) * tpp + ((uint32_t)i % R0) * m0;
+    const uint32_t x = fft.in_pos(t, i);
+    T fr[kEbFields], fi[kEbFields];
     if (active) {
-      T nr[6], ni[6];
-      if (morph == 1) accumulate_tensor<T, S, 1>(vox, a.mats, a.M, a.V, (uint64_t)row * a.nx + x, nr, ni);
-      else accumulate_tensor<T, S, 0>(vox, a.mats, a.M, a.V, (uint64_t)row * a.nx + x, nr, ni);
-      const T w = hann ? a.hx[x] * wyz : (T)1;
+      const uint64_t idx = (uint64_t)row * a.nx + x;
+      if (morph == 1) tensor_fields<T, S, 1>(vox, a.mats, a.M, a.V, idx, a.Q, rot, fr, fi);
+      else tensor_fields<T, S, 0>(vox, a.mats, a.M, a.V, idx, a.Q, rot, fr, fi);
+      const T w = hann ? a.hx[x] * wyz : wyz;
 #pragma unroll
-      for (int b = 0; b < 2; b++) {
-        cx<T> pv[3];
-        apply_tensor<T>(nr, ni, a.e[b].x, a.e[b].y, a.e[b].z, pv);
-#pragma unroll
-        for (int c = 0; c < 3; c++) vall[3 * b + c][i] = cscale<T>(pv[c], w);
+      for (int c = 0; c < kEbFields; c++) {
+        fr[c] *= w;
+        fi[c] *= w;
       }
     } else {
 #pragma unroll
-      for (int c = 0; c < 6; c++) vall[c][i] = mk<T>(0, 0);
+      for (int c = 0; c < kEbFields; c++) fr[c] = fi[c] = 0;
+    }
+#pragma unroll
+    for (int c = 0; c < kEbFields; c++) {
+      fre[c][i] = fr[c];
+      fim[c][i] = fi[c];
     }
   }
-  const LdsMap L{P.n, true};
-  const uint32_t RL = P.radix[P.nstages - 1];
 #pragma unroll
-  for (int c = 0; c < 6; c++) {
+  for (int c = 0; c < kEbFields; c++) {
     cx<T> v[E];
 #pragma unroll
-    for (int i = 0; i < E; i++) v[i] = vall[c][i];
-    stockham_from_regs<T, E>(v, P, t, tpp, p, lds, L, a.twx);
+    for (int i = 0; i < E; i++) v[i] = mk<T>(fre[c][i], fim[c][i]);
+    fft.run(v, t, p, lds, a.twx);
     if (active) {
       cx<T> *o = a.out[c] + (uint64_t)row * a.nx;
 #pragma unroll
-      for (int i = 0; i < E; i++) o[final_index(t, tpp, P.n, RL, i)] = v[i];
+      for (int i = 0; i < E; i++) o[fft.out_pos(t, i)] = v[i];
     }
+    // (run() ends with a barrier after its last LDS reads, so the next field's
+    // first LDS store cannot overtake them)
   }
 }
 
 // ---- Y FFT + z bins -----------------------------------------------------------
-template <typename T, int E, int NG>
-__global__ void __launch_bounds__(1024) ebasis_yz_kernel(StagePlan P, YzArgs<T> a) {
+// Block = pp columns (kx) x tpp threads, one field, one z chunk.  Per z slice the
+// next slice's loads are issued before the current slice's FFT.
+template <typename T, class F, int NG, bool THREE, int WG>
+__global__ void __launch_bounds__(WG) ebasis_yz_kernel(StagePlan P, YzArgs<T> a) {
+  constexpr int E = F::kE;
   extern __shared__ unsigned char smem_raw[];
   cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
+  const F fft(P, a.tpp, a.pp, false);
   const uint32_t pp = a.pp, tpp = a.tpp;
   const uint32_t p = threadIdx.x % pp, t = threadIdx.x / pp;
   const uint32_t kx = blockIdx.x * pp + p;
   const bool active = kx < a.nx;
   const uint32_t c = blockIdx.y;
-  const uint32_t zb = blockIdx.z * a.zchunk;
-  const uint32_t ze = zb + a.zchunk < a.nz ? zb + a.zchunk : a.nz;
   const uint64_t P2 = (uint64_t)a.nx * a.ny;
-  const uint32_t R0 = P.radix[0], m0 = P.n / R0;
-  const uint32_t RL = P.radix[P.nstages - 1];
-  int32_t kz[E][NG];
-  uint32_t ex[E][NG];
-  cx<T> acc[E][NG];
-#pragma unroll
-  for (int i = 0; i < E; i++) {
-    const uint32_t ky = final_index(t, tpp, P.n, RL, i);
-#pragma unroll
-    for (int q = 0; q < NG; q++) {
-      kz[i][q] = active ? a.kzt[(uint64_t)q * P2 + (uint64_t)ky * a.nx + kx] : -1;
-      ex[i][q] = kz[i][q] > 0 ? (uint32_t)(((uint64_t)zb * (uint32_t)kz[i][q]) % a.nz) : 0;
-      acc[i][q] = mk<T>(0, 0);
-    }
-  }
-  const LdsMap L{pp, false};
-  const cx<T> *vol = a.vol[c];
-  for (uint32_t z = zb; z < ze; z++) {
-    const cx<T> *base = vol + (uint64_t)z * P2 + kx;
+  const cx<T> *vol = a.vol[c] + (active ? kx : 0);
+
+  if constexpr (!THREE) {
+    // 2D: the only z bin is kz = 0; the Y FFT output is the projection input
     cx<T> v[E];
 #pragma unroll
     for (int i = 0; i < E; i++) {
-      const uint32_t yy = t + ((uint32_t)i / R0) * tpp + ((uint32_t)i % R0) * m0;
-      v[i] = active ? base[(uint64_t)yy * a.nx] : mk<T>(0, 0);
+      v[i] = active ? vol[(uint64_t)fft.in_pos(t, i) * a.nx] : mk<T>(0, 0);
     }
-    stockham_from_regs<T, E>(v, P, t, tpp, p, lds, L, a.twy);
+    fft.run(v, t, p, lds, a.twy);
+    if (active) {
 #pragma unroll
-    for (int i = 0; i < E; i++)
-#pragma unroll
-      for (int q = 0; q < NG; q++) {
-        if (kz[i][q] >= 0) {
-          const cx<T> w = a.nz > 1 ? a.twz[ex[i][q]] : mk<T>(1, 0);  // 2D: the only bin is kz = 0
-          acc[i][q].x += v[i].x * w.x - v[i].y * w.y;
-          acc[i][q].y += v[i].x * w.y + v[i].y * w.x;
-          uint32_t e = ex[i][q] + (uint32_t)kz[i][q];
-          ex[i][q] = e >= a.nz ? e - a.nz : e;
-        }
-      }
-  }
-  if (active) {
+      for (int i = 0; i < E; i++) a.part[(uint64_t)c * P2 + (uint64_t)fft.out_pos(t, i) * a.nx + kx] = v[i];
+    }
+    return;
+  } else {
+    cx<T> *ltw = lds + (size_t)pp * a.ny;  // z twiddles W^k, k < nz
+    for (uint32_t i = threadIdx.x; i < a.nz; i += blockDim.x) ltw[i] = a.twz[i];
+    const uint32_t zb = blockIdx.z * a.zchunk;
+    const uint32_t ze = zb + a.zchunk < a.nz ? zb + a.zchunk : a.nz;
+    // per element: (z-bin << 16) | running exponent z * bin mod Nz; 0xffffffff = no bin
+    uint32_t kx16[E];
+    cx<T> acc0[E], acc1[NG == 2 ? E : 1];
 #pragma unroll
     for (int i = 0; i < E; i++) {
-      const uint32_t ky = final_index(t, tpp, P.n, RL, i);
+      const uint32_t ky = fft.out_pos(t, i);
+      const int32_t b = active ? a.kzt[(uint64_t)ky * a.nx + kx] : -1;
+      kx16[i] = b < 0 ? 0xffffffffu : ((uint32_t)b << 16) | (uint32_t)(((uint64_t)zb * (uint32_t)b) % a.nz);
+      acc0[i] = mk<T>(0, 0);
+      if constexpr (NG == 2) acc1[i] = mk<T>(0, 0);
+    }
+    // slice z of this column group: uniform base + 32-bit lane offset + uniform element offsets
+    const uint32_t off0 = fft.in_pos(t, 0) * a.nx + (active ? kx : 0);
+    auto load = [&](uint32_t z, cx<T> (&dst)[E]) {
+      const cx<T> *zp = a.vol[c] + (uint64_t)z * P2;
 #pragma unroll
-      for (int q = 0; q < NG; q++)
-        a.part[(((uint64_t)blockIdx.z * NG + q) * 6 + c) * P2 + (uint64_t)ky * a.nx + kx] = acc[i][q];
+      for (int i = 0; i < E; i++) dst[i] = active ? zp[off0 + fft.in_delta(i) * a.nx] : mk<T>(0, 0);
+    };
+    cx<T> vn[E];
+    if (zb < ze) load(zb, vn);
+    lds_sync();  // z twiddles staged (the prefetch loads stay in flight)
+    for (uint32_t z = zb; z < ze; z++) {
+      cx<T> v[E];
+#pragma unroll
+      for (int i = 0; i < E; i++) v[i] = vn[i];
+      if (z + 1 < ze) load(z + 1, vn);
+      fft.run(v, t, p, lds, a.twy);
+      cx<T> wz = mk<T>(1, 0);  // W^(-z): second bin of the linear pair
+      if constexpr (NG == 2) wz = ltw[z == 0 ? 0 : a.nz - z];
+#pragma unroll
+      for (int i = 0; i < E; i++) {
+        const uint32_t pk = kx16[i];
+        if (pk != 0xffffffffu) {
+          const uint32_t ex = pk & 0xffffu;
+          const cx<T> w0 = ltw[ex];
+          const cx<T> u = mk<T>(v[i].x * w0.x - v[i].y * w0.y, v[i].x * w0.y + v[i].y * w0.x);
+          acc0[i].x += u.x;
+          acc0[i].y += u.y;
+          if constexpr (NG == 2) {
+            acc1[i].x += u.x * wz.x - u.y * wz.y;
+            acc1[i].y += u.x * wz.y + u.y * wz.x;
+          }
+          uint32_t e2 = ex + (pk >> 16);
+          e2 = e2 >= a.nz ? e2 - a.nz : e2;
+          kx16[i] = (pk & 0xffff0000u) | e2;
+        }
+      }
+    }
+    if (active) {
+#pragma unroll
+      for (int i = 0; i < E; i++) {
+        const uint64_t o = (uint64_t)fft.out_pos(t, i) * a.nx + kx;
+        a.part[(((uint64_t)blockIdx.z * NG + 0) * kEbFields + c) * P2 + o] = acc0[i];
+        if constexpr (NG == 2) a.part[(((uint64_t)blockIdx.z * NG + 1) * kEbFields + c) * P2 + o] = acc1[i];
+      }
     }
   }
 }
 
-// ---- DC replacement values from the X-transformed volumes ----------------------
-// F(kx,ky,kz) = sum_z sum_y Gx(kx,y,z) W_Ny^(y ky) W_Nz^(z kz) at the neighbours of
-// bin (0,0,0): (1,0,0), (Nx-1,0,0), (0,1,0), (0,Ny-1,0) and, in 3D, (0,0,1), (0,0,Nz-1).
+// ---- DC-replacement neighbours ------------------------------------------------
+// replaceDCComponentWithAverage (src/cudaMain.cu:123-176) averages F at (1,0,0),
+// (Nx-1,0,0), (0,1,0), (0,Ny-1,0) and, in 3D, (0,0,1), (0,0,Nz-1).  From the
+// X-transformed fields Gx their sum is
+//   sum_{y,z} Gx(1,y,z) + Gx(Nx-1,y,z) + Gx(0,y,z) (2cos(2 pi y/Ny) + 2cos(2 pi z/Nz)),
+// the last term only in 3D.  One block per (z, field) writes its FP64 slice sum.
 template <typename T>
-__global__ void __launch_bounds__(1024) ebasis_dcx_kernel(DcxArgs<T> a) {
-  __shared__ double red[1024];
-  const cx<T> *G = a.vol[blockIdx.x];
-  const uint64_t P2 = (uint64_t)a.nx * a.ny;
-  const uint64_t n = (uint64_t)a.ny * a.nz;
-  double s[12];
-#pragma unroll
-  for (int i = 0; i < 12; i++) s[i] = 0;
-  for (uint64_t r = threadIdx.x; r < n; r += blockDim.x) {
-    const uint32_t y = (uint32_t)(r % a.ny), z = (uint32_t)(r / a.ny);
-    const cx<T> *row = G + (uint64_t)z * P2 + (uint64_t)y * a.nx;
-    const cx<T> g1 = row[1], gm = row[a.nx - 1], g0 = row[0];
-    s[0] += (double)g1.x + (double)gm.x;
-    s[1] += (double)g1.y + (double)gm.y;
-    const cx<T> wy1 = a.twy[y], wy2 = a.twy[(a.ny - y) % a.ny];
-    const cx<T> a1 = cmul<T>(g0, wy1), a2 = cmul<T>(g0, wy2);
-    s[2] += (double)a1.x + (double)a2.x;
-    s[3] += (double)a1.y + (double)a2.y;
-    if (a.nz > 1) {
-      const cx<T> wz1 = a.twz[z], wz2 = a.twz[(a.nz - z) % a.nz];
-      const cx<T> b1 = cmul<T>(g0, wz1), b2 = cmul<T>(g0, wz2);
-      s[4] += (double)b1.x + (double)b2.x;
-      s[5] += (double)b1.y + (double)b2.y;
-    }
+__global__ void __launch_bounds__(256) ebasis_dc_kernel(DcArgs<T> a) {
+  __shared__ double red[2][256];
+  const uint32_t z = blockIdx.x, c = blockIdx.y;
+  const cx<T> *G = a.vol[c] + (uint64_t)z * a.nx * a.ny;
+  const double cz = a.nz > 1 ? 2.0 * (double)a.twz[z].x : 0.0;
+  double sr = 0, si = 0;
+  for (uint32_t y = threadIdx.x; y < a.ny; y += blockDim.x) {
+    const cx<T> *row = G + (uint64_t)y * a.nx;
+    const cx<T> g0 = row[0], g1 = row[1], gm = row[a.nx - 1];
+    const double w = 2.0 * (double)a.twy[y].x + cz;
+    sr += (double)g1.x + (double)gm.x + w * (double)g0.x;
+    si += (double)g1.y + (double)gm.y + w * (double)g0.y;
   }
-  double tot[6];
-  for (int k = 0; k < 6; k++) {
-    red[threadIdx.x] = s[k];
-    __syncthreads();
-    for (uint32_t w = blockDim.x / 2; w > 0; w >>= 1) {
-      if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-      __syncthreads();
+  red[0][threadIdx.x] = sr;
+  red[1][threadIdx.x] = si;
+  __syncthreads();
+  for (uint32_t s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      red[0][threadIdx.x] += red[0][threadIdx.x + s];
+      red[1][threadIdx.x] += red[1][threadIdx.x + s];
     }
-    tot[k] = red[0];
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    const double cnt = a.nz > 1 ? 6.0 : 4.0;
-    a.dcval[blockIdx.x] = mk<T>((T)((tot[0] + tot[2] + tot[4]) / cnt), (T)((tot[1] + tot[3] + tot[5]) / cnt));
+    double2 o;
+    o.x = red[0][0];
+    o.y = red[1][0];
+    a.dcz[(uint64_t)c * a.nz + z] = o;
   }
 }
 
 // ---- chunk sums, DC, and every angle's intensity -----------------------------
-template <typename T, int NG, bool MATFRAME>
+template <typename T, int NG>
 __global__ void __launch_bounds__(256) ebasis_angles_kernel(AnglesArgs<T> a) {
   const EwaldGeom &g = a.g;
   const uint64_t P2 = (uint64_t)g.nx * g.ny;
@@ -228,26 +325,35 @@ __global__ void __launch_bounds__(256) ebasis_angles_kernel(AnglesArgs<T> a) {
   const uint32_t ix = unshift(X, g.nx), iy = unshift(Y, g.ny);
   const uint64_t col = (uint64_t)iy * g.nx + ix;
   const bool three = !g.enable2d;
-  cx<T> F[NG][6];
+  cx<T> F[NG][kEbFields];
 #pragma unroll
   for (int q = 0; q < NG; q++)
 #pragma unroll
-    for (int c = 0; c < 6; c++) {
+    for (int c = 0; c < kEbFields; c++) {
       cx<T> s = mk<T>(0, 0);
       for (uint32_t ch = 0; ch < a.nchunks; ch++) {
-        const cx<T> v = a.part[(((uint64_t)ch * NG + q) * 6 + c) * P2 + col];
+        const cx<T> v = a.part[(((uint64_t)ch * NG + q) * kEbFields + c) * P2 + col];
         s.x += v.x;
         s.y += v.y;
       }
       F[q][c] = s;
     }
   if (ix == 0 && iy == 0) {
+    const double cnt = three ? 6.0 : 4.0;
 #pragma unroll
     for (int q = 0; q < NG; q++) {
       const uint32_t kzq = three ? unshift(z0 + q, g.nz) : 0;
-      if (kzq == 0)
+      if (kzq != 0) continue;
 #pragma unroll
-        for (int c = 0; c < 6; c++) F[q][c] = a.dcval[c];
+      for (int c = 0; c < kEbFields; c++) {
+        double sr = 0, si = 0;
+        for (uint32_t z = 0; z < g.nz; z++) {
+          const double2 v = a.dcz[(uint64_t)c * g.nz + z];
+          sr += v.x;
+          si += v.y;
+        }
+        F[q][c] = mk<T>((T)(sr / cnt), (T)(si / cnt));
+      }
     }
   }
   double md[NG][6];
@@ -255,25 +361,20 @@ __global__ void __launch_bounds__(256) ebasis_angles_kernel(AnglesArgs<T> a) {
   for (int q = 0; q < NG; q++) scatter_matrix(g, X, Y, three ? z0 + q : 0, md[q]);
   const T w1 = (T)f;
   for (uint32_t an = 0; an < a.nA; an++) {
-    const T cc = a.coef[2 * an], ss = a.coef[2 * an + 1];
+    const T *A = a.amat + (uint64_t)kEbAmat * an;
     T I[NG];
 #pragma unroll
     for (int q = 0; q < NG; q++) {
       cx<T> pv[3];
 #pragma unroll
-      for (int c = 0; c < 3; c++)
-        pv[c] = mk<T>(cc * F[q][c].x + ss * F[q][c + 3].x, cc * F[q][c].y + ss * F[q][c + 3].y);
-      if constexpr (MATFRAME) {
-        const T *R = a.rmat + 9 * (uint64_t)an;
-        cx<T> r[3];
+      for (int i = 0; i < 3; i++) {
+        T re = 0, im = 0;
 #pragma unroll
-        for (int i = 0; i < 3; i++) {
-          r[i].x = R[0 + i] * pv[0].x + R[3 + i] * pv[1].x + R[6 + i] * pv[2].x;
-          r[i].y = R[0 + i] * pv[0].y + R[3 + i] * pv[1].y + R[6 + i] * pv[2].y;
+        for (int j = 0; j < kEbFields; j++) {
+          re += A[i * kEbFields + j] * F[q][j].x;
+          im += A[i * kEbFields + j] * F[q][j].y;
         }
-        pv[0] = r[0];
-        pv[1] = r[1];
-        pv[2] = r[2];
+        pv[i] = mk<T>(re, im);
       }
       I[q] = scatter_intensity<T>(md[q], pv);
     }
@@ -287,16 +388,27 @@ __global__ void __launch_bounds__(256) ebasis_angles_kernel(AnglesArgs<T> a) {
 // Elements-per-thread instantiations: FP32 up to 24 (radix lcm x 2^k), FP64 up to 8.
 template <typename T> constexpr bool e_allowed(int E) { return sizeof(T) == 4 || E <= 8; }
 
+// Launch shapes of the compile-time plans: X rows of <= 64 threads share a wave;
+// Y tiles are PP columns wide (16 columns = 128 B rows of complex64).
+constexpr int ct_polx_wg(int N, int E) { return N / E <= 64 ? 64 : N / E; }
+constexpr int ct_yz_pp(int N) { return N <= 512 ? 16 : (N == 1024 ? 8 : 4); }
+uint32_t ct_yz_pp_of(uint32_t n) { return n <= 512 ? 16u : (n == 1024 ? 8u : 4u); }
+
+template <typename T, typename S, int E>
+hipError_t launch_polx_rt(dim3 grid, dim3 block, size_t lds, hipStream_t st, const StagePlan &P,
+                          const PolxArgs<T> &a, const void *vox, int morph, int hann) {
+  hipLaunchKernelGGL((ebasis_polx_kernel<T, S, RtFft<T, E>, 256>), grid, block, lds, st, P, a, (const S *)vox,
+                     morph, hann);
+  return hipGetLastError();
+}
+
 template <typename T, typename S>
-hipError_t launch_polx_e(uint32_t E, dim3 grid, dim3 block, size_t lds, hipStream_t st, const StagePlan &P,
-                         const PolxArgs<T> &a, const void *vox, int morph, int hann, bool *found) {
+hipError_t launch_polx_rt_e(uint32_t E, dim3 grid, dim3 block, size_t lds, hipStream_t st, const StagePlan &P,
+                            const PolxArgs<T> &a, const void *vox, int morph, int hann, bool *found) {
   *found = true;
-#define CYR_PX(EV)                                                                                            \
-  case EV:                                                                                                    \
-    if constexpr (e_allowed<T>(EV)) {                                                                         \
-      hipLaunchKernelGGL((ebasis_polx_kernel<T, S, EV>), grid, block, lds, st, P, a, (const S *)vox, morph, hann); \
-      return hipGetLastError();                                                                               \
-    }                                                                                                         \
+#define CYR_PX(EV)                                                                      \
+  case EV:                                                                              \
+    if constexpr (e_allowed<T>(EV)) return launch_polx_rt<T, S, EV>(grid, block, lds, st, P, a, vox, morph, hann); \
     break;
   switch (E) {
     CYR_PX(2)
@@ -316,16 +428,48 @@ hipError_t launch_polx_e(uint32_t E, dim3 grid, dim3 block, size_t lds, hipStrea
   return hipSuccess;
 }
 
-template <typename T, int NG>
-hipError_t launch_yz_e(uint32_t E, dim3 grid, dim3 block, size_t lds, hipStream_t st, const StagePlan &P,
-                       const YzArgs<T> &a, bool *found) {
+template <typename T, typename S, int N>
+hipError_t launch_polx_ct(dim3 grid, size_t lds, hipStream_t st, const StagePlan &P, const PolxArgs<T> &a,
+                          const void *vox, int morph, int hann) {
+  using F = typename CtPlan<T, N, true, 1>::type;
+  constexpr int WG = ct_polx_wg(N, F::kE);
+  hipLaunchKernelGGL((ebasis_polx_kernel<T, S, F, WG>), grid, dim3(WG), lds, st, P, a, (const S *)vox, morph, hann);
+  return hipGetLastError();
+}
+
+template <typename T, typename S>
+hipError_t launch_polx_ct_n(uint32_t n, dim3 grid, size_t lds, hipStream_t st, const StagePlan &P,
+                            const PolxArgs<T> &a, const void *vox, int morph, int hann, bool *found) {
   *found = true;
-#define CYR_YZ(EV)                                                                   \
-  case EV:                                                                           \
-    if constexpr (e_allowed<T>(EV)) {                                                \
-      hipLaunchKernelGGL((ebasis_yz_kernel<T, EV, NG>), grid, block, lds, st, P, a); \
-      return hipGetLastError();                                                      \
-    }                                                                                \
+  switch (n) {
+    case 64: return launch_polx_ct<T, S, 64>(grid, lds, st, P, a, vox, morph, hann);
+    case 128: return launch_polx_ct<T, S, 128>(grid, lds, st, P, a, vox, morph, hann);
+    case 256: return launch_polx_ct<T, S, 256>(grid, lds, st, P, a, vox, morph, hann);
+    case 384:
+      if constexpr (e_allowed<T>(12)) return launch_polx_ct<T, S, 384>(grid, lds, st, P, a, vox, morph, hann);
+      break;
+    case 512: return launch_polx_ct<T, S, 512>(grid, lds, st, P, a, vox, morph, hann);
+    case 1024: return launch_polx_ct<T, S, 1024>(grid, lds, st, P, a, vox, morph, hann);
+    case 2048: return launch_polx_ct<T, S, 2048>(grid, lds, st, P, a, vox, morph, hann);
+    default: break;
+  }
+  *found = false;
+  return hipSuccess;
+}
+
+template <int E> constexpr int rt_yz_wg() { return E <= 8 ? 1024 : 512; }
+
+template <typename T, int NG, bool THREE>
+hipError_t launch_yz_rt_e(uint32_t E, dim3 grid, dim3 block, size_t lds, hipStream_t st, const StagePlan &P,
+                          const YzArgs<T> &a, bool *found) {
+  *found = true;
+#define CYR_YZ(EV)                                                                                              \
+  case EV:                                                                                                      \
+    if constexpr (e_allowed<T>(EV)) {                                                                           \
+      hipLaunchKernelGGL((ebasis_yz_kernel<T, RtFft<T, EV>, NG, THREE, rt_yz_wg<EV>()>), grid, block, lds, st, P, \
+                         a);                                                                                    \
+      return hipGetLastError();                                                                                 \
+    }                                                                                                           \
     break;
   switch (E) {
     CYR_YZ(2)
@@ -345,13 +489,33 @@ hipError_t launch_yz_e(uint32_t E, dim3 grid, dim3 block, size_t lds, hipStream_
   return hipSuccess;
 }
 
-// z chunks of the Y/z kernel: enough blocks (x groups x 6 volumes x chunks) to
-// reach target_blocks, at most one chunk per z slice.
-uint32_t yz_chunks(uint32_t nx, uint32_t nz, uint32_t pp, uint32_t target_blocks) {
-  const uint32_t per = ceil_div(nx, pp) * 6;
-  uint32_t nch = target_blocks / per;
-  if (nch < 1) nch = 1;
-  return nch > nz ? nz : nch;
+template <typename T, int NG, bool THREE, int N>
+hipError_t launch_yz_ct(dim3 grid, size_t lds, hipStream_t st, const StagePlan &P, const YzArgs<T> &a) {
+  constexpr int PP = ct_yz_pp(N);
+  using F = typename CtPlan<T, N, false, PP>::type;
+  constexpr int WG = PP * F::TPP;
+  hipLaunchKernelGGL((ebasis_yz_kernel<T, F, NG, THREE, WG>), grid, dim3(WG), lds, st, P, a);
+  return hipGetLastError();
+}
+
+template <typename T, int NG, bool THREE>
+hipError_t launch_yz_ct_n(uint32_t n, dim3 grid, size_t lds, hipStream_t st, const StagePlan &P, const YzArgs<T> &a,
+                          bool *found) {
+  *found = true;
+  switch (n) {
+    case 64: return launch_yz_ct<T, NG, THREE, 64>(grid, lds, st, P, a);
+    case 128: return launch_yz_ct<T, NG, THREE, 128>(grid, lds, st, P, a);
+    case 256: return launch_yz_ct<T, NG, THREE, 256>(grid, lds, st, P, a);
+    case 384:
+      if constexpr (e_allowed<T>(12)) return launch_yz_ct<T, NG, THREE, 384>(grid, lds, st, P, a);
+      break;
+    case 512: return launch_yz_ct<T, NG, THREE, 512>(grid, lds, st, P, a);
+    case 1024: return launch_yz_ct<T, NG, THREE, 1024>(grid, lds, st, P, a);
+    case 2048: return launch_yz_ct<T, NG, THREE, 2048>(grid, lds, st, P, a);
+    default: break;
+  }
+  *found = false;
+  return hipSuccess;
 }
 
 // Elements per thread for a pencil of length n: the radix lcm, doubled while the
@@ -362,10 +526,19 @@ uint32_t choose_e(uint32_t n, uint32_t lcm, uint32_t tpp_cap, uint32_t e_cap) {
   return E;
 }
 
-}  // namespace
+// z chunks of the Y/z kernel: enough blocks (x groups x fields x chunks) to
+// reach target_blocks, at most one chunk per z slice.
+uint32_t yz_chunks(uint32_t nx, uint32_t nz, uint32_t pp, uint32_t target_blocks) {
+  const uint32_t per = ceil_div(nx, pp) * kEbFields;
+  uint32_t nch = target_blocks / per;
+  if (nch < 1) nch = 1;
+  return nch > nz ? nz : nch;
+}
 
-// Elements-per-thread values with a compiled kernel (launch_polx_e / launch_yz_e).
-static bool e_instantiated(uint32_t E, int precision) {
+uint32_t yz_wg_of(uint32_t E) { return E <= 8 ? 1024u : 512u; }
+
+// Elements-per-thread values with a compiled kernel (launch_polx_rt_e / launch_yz_rt_e).
+bool e_instantiated(uint32_t E, int precision) {
   switch (E) {
     case 2: case 3: case 4: case 5: case 6: case 8: return true;
     case 12: case 16: case 24: return precision != CYR_FP64;
@@ -373,20 +546,32 @@ static bool e_instantiated(uint32_t E, int precision) {
   }
 }
 
+// The compile-time plan applies (its elements per thread are compiled for T).
+bool ct_usable(uint32_t n, int precision) {
+  return ct_length(n) && (precision != CYR_FP64 || ct_elements(n) <= 8);
+}
+
+}  // namespace
+
 bool ebasis_supported(uint32_t nx, uint32_t ny, int precision) {
   StagePlan P;
   uint32_t lcm;
   const size_t cs = precision == CYR_FP64 ? 16 : 8;
   const uint32_t ecap = precision == CYR_FP64 ? 8 : 16;
-  if (!make_stage_plan(nx, P, lcm)) return false;
-  const uint32_t ex = choose_e(nx, lcm, 256, ecap);
-  if (!e_instantiated(ex, precision) || nx / ex > 256) return false;  // one X row per <= 256 threads
-  if (!make_stage_plan(ny, P, lcm)) return false;
-  // the Y kernel's widest case: one column per block (pp = 1)
-  const uint32_t ey = choose_e(ny, lcm, 1024, ecap);
-  if (!e_instantiated(ey, precision) || ny / ey > 1024) return false;
   const size_t lds = device_max_lds();
-  return (size_t)nx * cs <= lds && (size_t)ny * cs <= lds && nx >= 2 && ny >= 2;
+  if (nx < 2 || ny < 2) return false;
+  if (!ct_usable(nx, precision)) {  // run-time X plan: one row per <= 256 threads
+    if (!make_stage_plan(nx, P, lcm)) return false;
+    const uint32_t ex = choose_e(nx, lcm, 256, ecap);
+    if (!e_instantiated(ex, precision) || nx / ex > 256 || (size_t)nx * cs > lds) return false;
+  }
+  // the Y kernel's narrowest run-time block: one column (pp = 1); the z twiddles
+  // take at most 4096 entries of LDS
+  if (!make_stage_plan(ny, P, lcm)) return false;
+  const uint32_t ey = choose_e(ny, lcm, yz_wg_of(lcm), ecap);
+  if (ct_usable(ny, precision) && (size_t)ct_yz_pp_of(ny) * ny * cs + 4096 * cs <= lds) return true;
+  if (!e_instantiated(ey, precision) || ny / ey > yz_wg_of(ey)) return false;
+  return (size_t)ny * cs + 4096 * cs <= lds;
 }
 
 uint32_t ebasis_max_chunks(uint32_t nx, uint32_t nz) {
@@ -395,9 +580,9 @@ uint32_t ebasis_max_chunks(uint32_t nx, uint32_t nz) {
   return yz_chunks(nx, nz, 16, kYzTargetBlocks);
 }
 
-cyr_status launch_ebasis_bins(const EwaldGeom &g, int ng, int32_t *kzt, hipStream_t st) {
+cyr_status launch_ebasis_bins(const EwaldGeom &g, int32_t *kzt, hipStream_t st) {
   const uint64_t P = (uint64_t)g.nx * g.ny;
-  hipLaunchKernelGGL(ebasis_bins_kernel, dim3(ceil_div(P, 256)), dim3(256), 0, st, g, ng, kzt);
+  hipLaunchKernelGGL(ebasis_bins_kernel, dim3(ceil_div(P, 256)), dim3(256), 0, st, g, kzt);
   CYR_LAUNCHED();
   return CYR_OK;
 }
@@ -407,11 +592,16 @@ cyr_status launch_ebasis_polx(PolxArgs<T> a, const void *vox, int storage, int m
   StagePlan P;
   uint32_t lcm;
   if (!make_stage_plan(a.nx, P, lcm)) return fail(CYR_E_UNSUPPORTED, "E-basis: Nx must be 2^a 3^b 5^c");
-  const uint32_t E = choose_e(a.nx, lcm, 256, sizeof(T) == 8 ? 8 : 16);
+  const int prec = sizeof(T) == 8 ? CYR_FP64 : CYR_FP32;
+  const bool ct = ct_usable(a.nx, prec);
+  const uint32_t E = ct ? ct_elements(a.nx) : choose_e(a.nx, lcm, 256, sizeof(T) == 8 ? 8 : 16);
   a.tpp = a.nx / E;
   if (a.tpp > 256) return fail(CYR_E_UNSUPPORTED, "E-basis: X row needs more than 256 threads");
-  a.pp = a.tpp >= 256 ? 1 : 256 / a.tpp;
-  if (a.pp > a.rows) a.pp = a.rows;
+  // compile-time plans: rows of <= 64 threads share one wave; the run-time plan
+  // packs rows into 256-thread blocks
+  const uint32_t wg = ct ? (a.tpp <= 64 ? 64u : a.tpp) : 256u;
+  a.pp = wg / a.tpp;
+  if (a.pp < 1) a.pp = 1;
   const size_t lds = (size_t)a.pp * a.nx * sizeof(cx<T>);
   if (lds > device_max_lds()) return fail(CYR_E_UNSUPPORTED, "E-basis: row does not fit in LDS");
   cyr_status s;
@@ -420,16 +610,18 @@ cyr_status launch_ebasis_polx(PolxArgs<T> a, const void *vox, int storage, int m
   const dim3 grid(ceil_div(a.rows, a.pp)), block(a.pp * a.tpp);
   bool found = false;
   hipError_t e = hipSuccess;
+  auto go = [&](auto tag) {
+    using S = decltype(tag);
+    if (ct) e = launch_polx_ct_n<T, S>(a.nx, grid, lds, st, P, a, vox, morph, hann ? 1 : 0, &found);
+    else e = launch_polx_rt_e<T, S>(E, grid, block, lds, st, P, a, vox, morph, hann ? 1 : 0, &found);
+  };
   if constexpr (sizeof(T) == 4) {
-    if (storage == CYR_STORE_FP16)
-      e = launch_polx_e<T, half4>(E, grid, block, lds, st, P, a, vox, morph, hann ? 1 : 0, &found);
-    else if (storage == CYR_STORE_BF16)
-      e = launch_polx_e<T, bf164>(E, grid, block, lds, st, P, a, vox, morph, hann ? 1 : 0, &found);
-    else
-      e = launch_polx_e<T, float4>(E, grid, block, lds, st, P, a, vox, morph, hann ? 1 : 0, &found);
+    if (storage == CYR_STORE_FP16) go(half4{});
+    else if (storage == CYR_STORE_BF16) go(bf164{});
+    else go(float4{});
   } else {
     if (storage != CYR_STORE_NATIVE) return fail(CYR_E_UNSUPPORTED, "E-basis FP64: native morphology storage only");
-    e = launch_polx_e<T, double4>(E, grid, block, lds, st, P, a, vox, morph, hann ? 1 : 0, &found);
+    go(double4{});
   }
   if (e != hipSuccess) return hip_fail(e, "ebasis_polx_kernel", __FILE__, __LINE__);
   if (!found) return fail(CYR_E_UNSUPPORTED, "E-basis: no X kernel for this length");
@@ -440,17 +632,29 @@ template <typename T> cyr_status ebasis_yz_plan(YzArgs<T> &a, uint32_t target_bl
   StagePlan P;
   uint32_t lcm;
   if (!make_stage_plan(a.ny, P, lcm)) return fail(CYR_E_UNSUPPORTED, "E-basis: Ny must be 2^a 3^b 5^c");
-  uint32_t pp = a.pp ? a.pp : 16;
-  uint32_t E = a.e ? a.e : choose_e(a.ny, lcm, 1024 / pp, sizeof(T) == 8 ? 8 : 16);
-  if (E % lcm != 0 || a.ny % E != 0) E = lcm;
-  const uint32_t tpp = a.ny / E;
-  while (pp > 1 && ((size_t)pp * a.ny * sizeof(cx<T>) > device_max_lds() || pp * tpp > 1024)) pp /= 2;
-  if ((size_t)pp * a.ny * sizeof(cx<T>) > device_max_lds() || pp * tpp > 1024)
-    return fail(CYR_E_UNSUPPORTED, "E-basis: column does not fit in LDS");
-  a.e = E;
-  a.tpp = tpp;
-  a.pp = pp;
-  if (!a.zchunk) a.zchunk = ceil_div(a.nz, yz_chunks(a.nx, a.nz, pp, target_blocks));
+  const int prec = sizeof(T) == 8 ? CYR_FP64 : CYR_FP32;
+  const size_t tw = a.nz > 1 ? (size_t)a.nz * sizeof(cx<T>) : 0;
+  auto lds_of = [&](uint32_t q) { return (size_t)q * a.ny * sizeof(cx<T>) + tw; };
+  a.ct = 0;
+  if (!a.e && !a.pp && ct_usable(a.ny, prec) && lds_of(ct_yz_pp_of(a.ny)) <= device_max_lds()) {
+    a.ct = 1;
+    a.e = ct_elements(a.ny);
+    a.pp = ct_yz_pp_of(a.ny);
+    a.tpp = a.ny / a.e;
+  } else {
+    const uint32_t ecap = sizeof(T) == 8 ? 8 : 16;
+    uint32_t pp = a.pp ? a.pp : 16;
+    uint32_t E = a.e ? a.e : choose_e(a.ny, lcm, yz_wg_of(lcm) / pp, ecap);
+    if (E % lcm != 0 || a.ny % E != 0 || !e_instantiated(E, prec)) E = lcm;
+    const uint32_t tpp = a.ny / E;
+    const uint32_t wg = yz_wg_of(E);
+    while (pp > 1 && (lds_of(pp) > device_max_lds() || pp * tpp > wg)) pp /= 2;
+    if (lds_of(pp) > device_max_lds() || pp * tpp > wg) return fail(CYR_E_UNSUPPORTED, "E-basis: column does not fit");
+    a.e = E;
+    a.tpp = tpp;
+    a.pp = pp;
+  }
+  if (!a.zchunk) a.zchunk = ceil_div(a.nz, yz_chunks(a.nx, a.nz, a.pp, target_blocks));
   a.nchunks = ceil_div(a.nz, a.zchunk);
   return CYR_OK;
 }
@@ -458,35 +662,52 @@ template <typename T> cyr_status ebasis_yz_plan(YzArgs<T> &a, uint32_t target_bl
 template <typename T> cyr_status launch_ebasis_yz(YzArgs<T> a, int ng, hipStream_t st) {
   StagePlan P;
   uint32_t lcm;
-  if (!make_stage_plan(a.ny, P, lcm) || !a.e || !a.pp || !a.tpp || !a.zchunk || a.ny % a.e != 0 || a.e % lcm != 0)
+  if (!make_stage_plan(a.ny, P, lcm) || !a.e || !a.pp || !a.tpp || !a.zchunk || a.ny % a.e != 0 || a.e % lcm != 0 ||
+      a.tpp != a.ny / a.e)
     return fail(CYR_E_INVALID, "E-basis: Y plan missing (call ebasis_yz_plan)");
+  const int prec = sizeof(T) == 8 ? CYR_FP64 : CYR_FP32;
+  if (a.ct && (!ct_usable(a.ny, prec) || a.e != ct_elements(a.ny) || a.pp != ct_yz_pp_of(a.ny)))
+    return fail(CYR_E_INVALID, "E-basis: inconsistent compile-time Y plan");
+  if (!a.ct && a.pp * a.tpp > yz_wg_of(a.e)) return fail(CYR_E_INVALID, "E-basis: Y block larger than its launch bound");
+  const bool three = a.nz > 1;
+  if (three && (!a.twz || !a.kzt)) return fail(CYR_E_INVALID, "E-basis: z tables missing");
+  if (three && a.nz > 0xffffu) return fail(CYR_E_UNSUPPORTED, "E-basis: Nz above 65535");
+  if (!three && (a.nchunks != 1 || ng != 1)) return fail(CYR_E_INVALID, "E-basis: 2D plan must be one chunk");
   cyr_status s;
   a.twy = twiddles<T>(a.ny, &s);
   if (!a.twy) return s;
   const uint32_t groups = ceil_div(a.nx, a.pp);
-  const dim3 grid(groups, 6, a.nchunks), block(a.pp * a.tpp);
-  const size_t lds = (size_t)a.pp * a.ny * sizeof(cx<T>);
+  const dim3 grid(groups, kEbFields, a.nchunks), block(a.pp * a.tpp);
+  const size_t lds = (size_t)a.pp * a.ny * sizeof(cx<T>) + (three ? (size_t)a.nz * sizeof(cx<T>) : 0);
+  if (lds > device_max_lds()) return fail(CYR_E_UNSUPPORTED, "E-basis: Y tile does not fit in LDS");
   bool found = false;
-  hipError_t e = ng == 2 ? launch_yz_e<T, 2>(a.e, grid, block, lds, st, P, a, &found)
-                         : launch_yz_e<T, 1>(a.e, grid, block, lds, st, P, a, &found);
+  hipError_t e;
+  if (a.ct) {
+    if (!three) e = launch_yz_ct_n<T, 1, false>(a.ny, grid, lds, st, P, a, &found);
+    else if (ng == 2) e = launch_yz_ct_n<T, 2, true>(a.ny, grid, lds, st, P, a, &found);
+    else e = launch_yz_ct_n<T, 1, true>(a.ny, grid, lds, st, P, a, &found);
+  } else {
+    if (!three) e = launch_yz_rt_e<T, 1, false>(a.e, grid, block, lds, st, P, a, &found);
+    else if (ng == 2) e = launch_yz_rt_e<T, 2, true>(a.e, grid, block, lds, st, P, a, &found);
+    else e = launch_yz_rt_e<T, 1, true>(a.e, grid, block, lds, st, P, a, &found);
+  }
   if (e != hipSuccess) return hip_fail(e, "ebasis_yz_kernel", __FILE__, __LINE__);
   if (!found) return fail(CYR_E_UNSUPPORTED, "E-basis: no Y kernel for this length");
   return CYR_OK;
 }
 
-template <typename T> cyr_status launch_ebasis_dcx(const DcxArgs<T> &a, hipStream_t st) {
-  hipLaunchKernelGGL((ebasis_dcx_kernel<T>), dim3(6), dim3(1024), 0, st, a);
+template <typename T> cyr_status launch_ebasis_dc(const DcArgs<T> &a, hipStream_t st) {
+  if (a.nx < 2 || a.ny < 2 || !a.twy || (a.nz > 1 && !a.twz)) return fail(CYR_E_INVALID, "E-basis: DC tables missing");
+  hipLaunchKernelGGL((ebasis_dc_kernel<T>), dim3(a.nz, kEbFields), dim3(256), 0, st, a);
   CYR_LAUNCHED();
   return CYR_OK;
 }
 
-template <typename T> cyr_status launch_ebasis_angles(const AnglesArgs<T> &a, int ng, bool matframe, hipStream_t st) {
+template <typename T> cyr_status launch_ebasis_angles(const AnglesArgs<T> &a, int ng, hipStream_t st) {
   const uint64_t P = (uint64_t)a.g.nx * a.g.ny;
   const dim3 grid(ceil_div(P, 256)), block(256);
-  if (ng == 2 && matframe) hipLaunchKernelGGL((ebasis_angles_kernel<T, 2, true>), grid, block, 0, st, a);
-  else if (ng == 2) hipLaunchKernelGGL((ebasis_angles_kernel<T, 2, false>), grid, block, 0, st, a);
-  else if (matframe) hipLaunchKernelGGL((ebasis_angles_kernel<T, 1, true>), grid, block, 0, st, a);
-  else hipLaunchKernelGGL((ebasis_angles_kernel<T, 1, false>), grid, block, 0, st, a);
+  if (ng == 2) hipLaunchKernelGGL((ebasis_angles_kernel<T, 2>), grid, block, 0, st, a);
+  else hipLaunchKernelGGL((ebasis_angles_kernel<T, 1>), grid, block, 0, st, a);
   CYR_LAUNCHED();
   return CYR_OK;
 }
@@ -495,8 +716,8 @@ template <typename T> cyr_status launch_ebasis_angles(const AnglesArgs<T> &a, in
   template cyr_status launch_ebasis_polx<T>(PolxArgs<T>, const void *, int, int, bool, hipStream_t);        \
   template cyr_status ebasis_yz_plan<T>(YzArgs<T> &, uint32_t);                                              \
   template cyr_status launch_ebasis_yz<T>(YzArgs<T>, int, hipStream_t);                                      \
-  template cyr_status launch_ebasis_dcx<T>(const DcxArgs<T> &, hipStream_t);                                \
-  template cyr_status launch_ebasis_angles<T>(const AnglesArgs<T> &, int, bool, hipStream_t);
+  template cyr_status launch_ebasis_dc<T>(const DcArgs<T> &, hipStream_t);                                   \
+  template cyr_status launch_ebasis_angles<T>(const AnglesArgs<T> &, int, hipStream_t);
 CYR_INST(float)
 CYR_INST(double)
 #undef CYR_INST

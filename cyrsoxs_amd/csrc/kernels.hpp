@@ -60,14 +60,20 @@ template <typename T>
 cyr_status launch_igor_shift(cx<T> *vol, uint32_t nx, uint32_t ny, uint32_t nz, hipStream_t st);
 
 // ---- E-basis executor (ebasis.hip) -----------------------------------------
+// Tensor fields transformed per (energy, k): N'00, N'01, N'02, N'11, N'12 of
+// N' = Q^T N Q, Q = [e1 e2 k^]; per-angle coefficient matrix A (3 x 5) with
+// p~ = A F.
+constexpr int kEbFields = 5;
+constexpr int kEbAmat = 3 * kEbFields;
 // True when both in-plane axis lengths have a Stockham plan that fits the fused kernels.
 bool ebasis_supported(uint32_t nx, uint32_t ny, int precision);
 // Grid size the Y/z kernel's z chunking aims for, and the most chunks any plan
 // for (nx, nz) uses (sizes the partial-sum buffer).
 constexpr uint32_t kYzTargetBlocks = 1024;
 uint32_t ebasis_max_chunks(uint32_t nx, uint32_t nz);
-// Pixel -> FFT z bin of its (kx, ky) column: kzt[q][ky][kx] (-1 = not sampled).
-cyr_status launch_ebasis_bins(const EwaldGeom &g, int ng, int32_t *kzt, hipStream_t st);
+// Pixel -> FFT z bin of its (kx, ky) column: kzt[ky][kx] (-1 = not sampled); with
+// linear interpolation the upper bin is kzt - 1 (mod Nz).
+cyr_status launch_ebasis_bins(const EwaldGeom &g, int32_t *kzt, hipStream_t st);
 
 template <typename T> struct PolxArgs {
   const MatConst<T> *mats;
@@ -75,46 +81,48 @@ template <typename T> struct PolxArgs {
   uint64_t V;
   uint32_t nx, ny, nz, rows;       // rows = ny * nz
   const T *hx, *hy, *hz;           // Hanning tables
-  Vec3<T> e[2];                    // E basis
+  Mat3<T> Q;                       // columns e1, e2, k^ (row-major)
+  int32_t rot;                     // 0 when Q is the identity
   const cx<T> *twx;                // filled by the launcher
   uint32_t tpp, pp;                // filled by the launcher
-  cx<T> *out[6];                   // X-transformed volumes (basis 1 xyz, basis 2 xyz)
+  cx<T> *out[kEbFields];           // X-transformed tensor fields
 };
 template <typename T>
 cyr_status launch_ebasis_polx(PolxArgs<T> a, const void *vox, int storage, int morph, bool hann, hipStream_t st);
 
 template <typename T> struct YzArgs {
-  const cx<T> *vol[6];             // X-transformed volumes
-  cx<T> *part;                     // [nchunks][NG][6][Ny][Nx] partial sums
-  const int32_t *kzt;              // [NG][Ny][Nx]
+  const cx<T> *vol[kEbFields];     // X-transformed fields
+  cx<T> *part;                     // [nchunks][NG][kEbFields][Ny][Nx] partial sums (2D: [kEbFields][Ny][Nx])
+  const int32_t *kzt;              // [Ny][Nx] (3D)
   const cx<T> *twy, *twz;          // twy filled by the launcher; twz null in 2D
   uint32_t nx, ny, nz;
-  uint32_t e, pp, tpp, zchunk, nchunks;  // 0 = automatic (launcher fills)
+  uint32_t e, pp, tpp, zchunk, nchunks;  // 0 = automatic (ebasis_yz_plan fills)
+  uint32_t ct;                     // 1: compile-time FFT plan for Ny (set by ebasis_yz_plan)
 };
 // Fills e, pp, tpp, zchunk, nchunks (fields left 0 are chosen automatically;
 // target_blocks is the grid size the z chunking aims for).
 template <typename T> cyr_status ebasis_yz_plan(YzArgs<T> &a, uint32_t target_blocks);
 template <typename T> cyr_status launch_ebasis_yz(YzArgs<T> a, int ng, hipStream_t st);
 
-template <typename T> struct DcxArgs {
-  const cx<T> *vol[6];
-  const cx<T> *twy, *twz;
+// DC-replacement neighbour sums per (field, z) slice, FP64.
+template <typename T> struct DcArgs {
+  const cx<T> *vol[kEbFields];     // X-transformed fields
+  const cx<T> *twy, *twz;          // twz null in 2D
   uint32_t nx, ny, nz;
-  cx<T> *dcval;                    // [6]
+  double2 *dcz;                    // [kEbFields][Nz]
 };
-template <typename T> cyr_status launch_ebasis_dcx(const DcxArgs<T> &a, hipStream_t st);
+template <typename T> cyr_status launch_ebasis_dc(const DcArgs<T> &a, hipStream_t st);
 
 template <typename T> struct AnglesArgs {
   EwaldGeom g;
   const cx<T> *part;
   uint32_t nchunks;
-  const cx<T> *dcval;
-  const T *coef;                   // [nA][2]
-  const T *rmat;                   // [nA][9] (MATERIAL frame)
+  const double2 *dcz;
+  const T *amat;                   // [nA][kEbAmat]
   uint32_t nA;
   T *proj;                         // [nA][Ny][Nx]
 };
-template <typename T> cyr_status launch_ebasis_angles(const AnglesArgs<T> &a, int ng, bool matframe, hipStream_t st);
+template <typename T> cyr_status launch_ebasis_angles(const AnglesArgs<T> &a, int ng, hipStream_t st);
 
 // ---- rotation / averaging (rotate.hip) --------------------------------------
 // Plain path: rotate one projection by the E angle and accumulate (FP64).

@@ -68,9 +68,53 @@ template <typename T> MatConst<T> mat_const(const double *m4) {
 
 // Per-simulation constant tables, uploaded once.
 struct Tables {
-  size_t mats = 0, han = 0, coef = 0, rmat = 0, rotinv = 0, detinv = 0, total = 0;
+  size_t mats = 0, han = 0, amat = 0, rotinv = 0, detinv = 0, total = 0;
   std::vector<unsigned char> host;
 };
+
+// Frame of the E-basis tensor fields: columns e1, e2, e1 x e2 (= k^), row-major.
+M3 ebasis_frame(const SimGeometry &g, uint32_t k) {
+  const V3 a = g.e1[k], b = g.e2[k];
+  const V3 c{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+  M3 q;
+  const double col[3][3] = {{a.x, a.y, a.z}, {b.x, b.y, b.z}, {c.x, c.y, c.z}};
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) q.m[i * 3 + j] = col[j][i];
+  return q;
+}
+
+bool is_identity(const M3 &q) {
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++)
+      if (q.m[i * 3 + j] != (i == j ? 1.0 : 0.0)) return false;
+  return true;
+}
+
+// Per (k, angle): A (3 x 5) with p~ = A F for the tensor fields F = (N'00, N'01,
+// N'02, N'11, N'12): p' = N' (c, s, 0)^T, p = Q p', and p <- R^T p in the
+// MATERIAL frame (include/uniaxial.h:163-165).
+void ebasis_amat(const SimGeometry &g, uint32_t k, uint32_t a, bool matframe, double A[kEbAmat]) {
+  const M3 Q = ebasis_frame(g, k);
+  const M3 &R = g.R[(size_t)k * g.nA + a];
+  double M[9];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      double v = 0;
+      if (matframe)
+        for (int l = 0; l < 3; l++) v += R.m[l * 3 + i] * Q.m[l * 3 + j];
+      else
+        v = Q.m[i * 3 + j];
+      M[i * 3 + j] = v;
+    }
+  const double cc = g.ebasis[((size_t)k * g.nA + a) * 2 + 0], ss = g.ebasis[((size_t)k * g.nA + a) * 2 + 1];
+  const double C[3][kEbFields] = {{cc, ss, 0, 0, 0}, {0, cc, 0, ss, 0}, {0, 0, cc, 0, ss}};
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < kEbFields; j++) {
+      double v = 0;
+      for (int l = 0; l < 3; l++) v += M[i * 3 + l] * C[l][j];
+      A[i * kEbFields + j] = v;
+    }
+}
 
 template <typename T>
 void build_tables(const cyr_sim_desc *d, const SimGeometry &g, const double *mats, Tables &tb) {
@@ -79,10 +123,8 @@ void build_tables(const cyr_sim_desc *d, const SimGeometry &g, const double *mat
   off = align256(off + sizeof(MatConst<T>) * g.nE * g.M);
   tb.han = off;
   off = align256(off + sizeof(T) * (g.nx + g.ny + g.nz));
-  tb.coef = off;
-  off = align256(off + sizeof(T) * 2 * g.nK * g.nA);
-  tb.rmat = off;
-  off = align256(off + sizeof(T) * 9 * g.nK * g.nA);
+  tb.amat = off;
+  off = align256(off + sizeof(T) * kEbAmat * g.nK * g.nA);
   tb.rotinv = off;
   off = align256(off + sizeof(double) * 6 * g.nK * g.nA);
   tb.detinv = off;
@@ -98,14 +140,15 @@ void build_tables(const cyr_sim_desc *d, const SimGeometry &g, const double *mat
   for (uint32_t i = 0; i < g.ny; i++) h[g.nx + i] = (T)(0.5 * (1 - std::cos(2 * M_PI * i / (g.ny))));
   for (uint32_t i = 0; i < g.nz; i++)
     h[g.nx + g.ny + i] = g.nz == 1 ? (T)1 : (T)(0.5 * (1 - std::cos(2 * M_PI * i / (g.nz))));
-  T *coef = reinterpret_cast<T *>(tb.host.data() + tb.coef);
-  for (size_t i = 0; i < (size_t)2 * g.nK * g.nA; i++) coef[i] = (T)g.ebasis[i];
-  T *rm = reinterpret_cast<T *>(tb.host.data() + tb.rmat);
-  for (size_t i = 0; i < (size_t)g.nK * g.nA; i++)
-    for (int j = 0; j < 9; j++) rm[9 * i + j] = (T)g.R[i].m[j];
+  T *am = reinterpret_cast<T *>(tb.host.data() + tb.amat);
+  for (uint32_t k = 0; k < g.nK; k++)
+    for (uint32_t a = 0; a < g.nA; a++) {
+      double A[kEbAmat];
+      ebasis_amat(g, k, a, d->reference_frame == 0, A);
+      for (int j = 0; j < kEbAmat; j++) am[((size_t)k * g.nA + a) * kEbAmat + j] = (T)A[j];
+    }
   std::memcpy(tb.host.data() + tb.rotinv, g.rot_inv.data(), sizeof(double) * 6 * g.nK * g.nA);
   std::memcpy(tb.host.data() + tb.detinv, g.det_inv.data(), sizeof(double) * 6 * g.nK);
-  (void)d;
 }
 
 template <typename T> Vec3<T> v3(const V3 &v) { return Vec3<T>{(T)v.x, (T)v.y, (T)v.z}; }
@@ -134,15 +177,14 @@ cyr_status run_impl(const cyr_sim_desc *d, const SimGeometry &g, const void *d_v
   unsigned char *tbase = reinterpret_cast<unsigned char *>(wb.tables.ptr);
   const MatConst<T> *d_mats = reinterpret_cast<const MatConst<T> *>(tbase + tb.mats);
   const T *d_han = reinterpret_cast<const T *>(tbase + tb.han);
-  const T *d_coef = reinterpret_cast<const T *>(tbase + tb.coef);
-  const T *d_rmat = reinterpret_cast<const T *>(tbase + tb.rmat);
+  const T *d_amat = reinterpret_cast<const T *>(tbase + tb.amat);
   const double *d_rotinv = reinterpret_cast<const double *>(tbase + tb.rotinv);
 
-  const uint32_t ncomp = ebasis ? 6 : 3;
+  const uint32_t ncomp = ebasis ? kEbFields : 3;
   CYR_TRY(ensure_buffer(wb.pol, sizeof(cx<T>) * V * ncomp));
   cx<T> *pol = reinterpret_cast<cx<T> *>(wb.pol.ptr);
-  cx<T> *vols[6];
-  void *vptr[6];
+  cx<T> *vols[kEbFields];
+  void *vptr[kEbFields];
   for (uint32_t c = 0; c < ncomp; c++) {
     vols[c] = pol + (uint64_t)c * V;
     vptr[c] = vols[c];
@@ -153,12 +195,11 @@ cyr_status run_impl(const cyr_sim_desc *d, const SimGeometry &g, const void *d_v
   CYR_TRY(ensure_buffer(wb.acc, sizeof(double) * P));
   CYR_TRY(ensure_buffer(wb.count, sizeof(uint32_t) * P));
   CYR_TRY(ensure_buffer(wb.avg, sizeof(double) * P));
-  CYR_TRY(ensure_buffer(wb.dc, sizeof(cx<T>) * 8));
+
   T *proj = reinterpret_cast<T *>(wb.proj.ptr);
   double *acc = reinterpret_cast<double *>(wb.acc.ptr);
   uint32_t *count = reinterpret_cast<uint32_t *>(wb.count.ptr);
   double *avg = reinterpret_cast<double *>(wb.avg.ptr);
-  cx<T> *dcval = reinterpret_cast<cx<T> *>(wb.dc.ptr);
   cyr_status s;
   const cx<T> *twz = g.nz > 1 ? twiddles<T>(g.nz, &s) : nullptr;
   if (g.nz > 1 && !twz) return s;
@@ -168,22 +209,25 @@ cyr_status run_impl(const cyr_sim_desc *d, const SimGeometry &g, const void *d_v
   const int ng = (g.nz > 1 && d->interpolation == 1) ? 2 : 1;
   int32_t *kzt = nullptr;
   cx<T> *part = nullptr;
+  double2 *dcz = nullptr;
   YzArgs<T> yz{};
   if (ebasis) {
     yz.nx = g.nx;
     yz.ny = g.ny;
     yz.nz = g.nz;
     CYR_TRY(ebasis_yz_plan<T>(yz, kYzTargetBlocks));
-    CYR_TRY(ensure_buffer(wb.bins, sizeof(int32_t) * P * ng));
-    CYR_TRY(ensure_buffer(wb.part, sizeof(cx<T>) * P * ng * 6 * yz.nchunks));
+    CYR_TRY(ensure_buffer(wb.bins, sizeof(int32_t) * P));
+    CYR_TRY(ensure_buffer(wb.part, sizeof(cx<T>) * P * ng * kEbFields * yz.nchunks));
+    CYR_TRY(ensure_buffer(wb.dc, sizeof(double2) * kEbFields * g.nz));
     kzt = reinterpret_cast<int32_t *>(wb.bins.ptr);
     part = reinterpret_cast<cx<T> *>(wb.part.ptr);
+    dcz = reinterpret_cast<double2 *>(wb.dc.ptr);
   }
   // algorithmic HBM bytes of the timed launches (cyr_kernel_timing)
   const double cxb = (double)sizeof(cx<T>);
   const double vox_el = d->storage == CYR_STORE_NATIVE ? (double)sizeof(T) : 2.0;
   const double vox_bytes = 4.0 * vox_el * g.M * V;
-  const double part_bytes = cxb * P * ng * 6 * yz.nchunks;
+  const double part_bytes = cxb * P * ng * kEbFields * yz.nchunks;
 
   PolarArgs<T> pa{};
   pa.M = g.M;
@@ -205,10 +249,10 @@ cyr_status run_impl(const cyr_sim_desc *d, const SimGeometry &g, const void *d_v
       const EwaldGeom geo = make_ewald_geom(g.nx, g.ny, g.nz, d->phys_size, g.kmag[e], kv, d->interpolation);
       T *out = d_out + ((uint64_t)e * g.nK + k) * P;
       if (ebasis) {
-        // bins -> polarization + X FFT -> DC values -> Y FFT + z bins -> angles -> rotation
+        // bins -> tensor fields + X FFT -> Y FFT + z bins (+ DC probes) -> angles -> rotation
         kt_begin(st);
-        CYR_TRY(launch_ebasis_bins(geo, ng, kzt, st));
-        kt_end("ebasis_bins", st, (double)P * ng * 4);
+        CYR_TRY(launch_ebasis_bins(geo, kzt, st));
+        kt_end("ebasis_bins", st, (double)P * 4);
         PolxArgs<T> xa{};
         xa.mats = pa.mats;
         xa.M = g.M;
@@ -220,42 +264,42 @@ cyr_status run_impl(const cyr_sim_desc *d, const SimGeometry &g, const void *d_v
         xa.hx = pa.hx;
         xa.hy = pa.hy;
         xa.hz = pa.hz;
-        xa.e[0] = v3<T>(g.e1[k]);
-        xa.e[1] = v3<T>(g.e2[k]);
-        for (int c = 0; c < 6; c++) xa.out[c] = vols[c];
+        const M3 Q = ebasis_frame(g, k);
+        xa.Q = m3<T>(Q);
+        xa.rot = is_identity(Q) ? 0 : 1;
+        for (int c = 0; c < kEbFields; c++) xa.out[c] = vols[c];
         kt_begin(st);
         CYR_TRY(launch_ebasis_polx<T>(xa, d_vox, d->storage, d->morphology_type, hann, st));
-        kt_end("ebasis_polx", st, vox_bytes + 6.0 * V * cxb);
-        DcxArgs<T> da{};
-        for (int c = 0; c < 6; c++) da.vol[c] = vols[c];
+        kt_end("ebasis_polx", st, vox_bytes + kEbFields * (double)V * cxb);
+        DcArgs<T> da{};
+        for (int c = 0; c < kEbFields; c++) da.vol[c] = vols[c];
         da.twy = twy;
         da.twz = twz;
         da.nx = g.nx;
         da.ny = g.ny;
         da.nz = g.nz;
-        da.dcval = dcval;
+        da.dcz = dcz;
         kt_begin(st);
-        CYR_TRY(launch_ebasis_dcx<T>(da, st));
-        kt_end("ebasis_dcx", st);
+        CYR_TRY(launch_ebasis_dc<T>(da, st));
+        kt_end("ebasis_dc", st, 3.0 * kEbFields * (double)g.ny * g.nz * cxb);
         YzArgs<T> ya = yz;
-        for (int c = 0; c < 6; c++) ya.vol[c] = vols[c];
+        for (int c = 0; c < kEbFields; c++) ya.vol[c] = vols[c];
         ya.part = part;
-        ya.kzt = kzt;
+        ya.kzt = g.nz > 1 ? kzt : nullptr;
         ya.twz = twz;
         kt_begin(st);
         CYR_TRY(launch_ebasis_yz<T>(ya, ng, st));
-        kt_end("ebasis_yz", st, 6.0 * V * cxb + (double)ng * P * 4 + part_bytes);
+        kt_end("ebasis_yz", st, kEbFields * (double)V * cxb + (g.nz > 1 ? (double)P * 4 : 0.0) + part_bytes);
         AnglesArgs<T> aa{};
         aa.g = geo;
         aa.part = part;
         aa.nchunks = yz.nchunks;
-        aa.dcval = dcval;
-        aa.coef = d_coef + (size_t)2 * k * g.nA;
-        aa.rmat = d_rmat + (size_t)9 * k * g.nA;
+        aa.dcz = dcz;
+        aa.amat = d_amat + (size_t)kEbAmat * k * g.nA;
         aa.nA = g.nA;
         aa.proj = proj;
         kt_begin(st);
-        CYR_TRY(launch_ebasis_angles<T>(aa, ng, matframe, st));
+        CYR_TRY(launch_ebasis_angles<T>(aa, ng, st));
         kt_end("ebasis_angles", st, part_bytes + (double)g.nA * P * sizeof(T));
         kt_begin(st);
         CYR_TRY(launch_rotate_all<T>(proj, d_rotinv + (size_t)6 * k * g.nA, g.nA, g.nx, g.ny, d->rot_mask, avg, st));
@@ -370,9 +414,10 @@ uint64_t work_bytes(const cyr_sim_desc *d, const SimGeometry &g) {
   // the fused E-basis kernels need 2^a 3^b 5^c in-plane lengths; others run the plain executor
   const bool ebasis = d->pipeline == CYR_PIPE_EBASIS && ebasis_supported(g.nx, g.ny, d->precision) &&
                       !(d->precision == CYR_FP64 && d->storage != CYR_STORE_NATIVE);
-  uint64_t b = cs * g.V * (ebasis ? 6 : 3);
-  if (ebasis) {  // pixel -> bin table and the Y/z partial sums (chunks x 2 bins x 6 volumes)
-    b += 4 * g.P * 2 + cs * g.P * 2 * 6 * ebasis_max_chunks(g.nx, g.nz);
+  uint64_t b = cs * g.V * (ebasis ? kEbFields : 3);
+  if (ebasis) {  // pixel -> bin table, Y/z partial sums (chunks x 2 bins x fields), DC probes
+    const uint64_t nch = ebasis_max_chunks(g.nx, g.nz);
+    b += 4 * g.P + cs * g.P * 2 * kEbFields * nch + 16 * kEbFields * g.nz;
   }
   if (!ebasis && d->algorithm == 1) b += cs * g.V * 6;
   if (!ebasis && d->scatter_approach == 1) b += rs * g.V;

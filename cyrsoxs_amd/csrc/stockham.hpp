@@ -26,6 +26,16 @@ struct StagePlan {
   uint32_t tstep[kMaxStages];
 };
 
+// Work-group barrier that orders LDS accesses only: outstanding global loads and
+// stores stay in flight across it (a prefetched slice, or the previous field's
+// stores, overlap the next stage).  In a single-wave work group the compiler
+// drops the s_barrier and keeps the LDS wait.
+__device__ inline void lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // ---- radix butterflies (forward, W = exp(-2 pi i / R)) ----------------------
 
 template <typename T> __device__ inline void bfly2(cx<T> *a) {
@@ -176,7 +186,7 @@ __device__ inline void stockham_from_regs(cx<T> (&v)[E], const StagePlan &P, uin
         case 5: if constexpr (E % 5 == 0) stage_load_lds<T, E, 5>(v, t, tpp, m, p, lds, L); break;
         default: if constexpr (E % 8 == 0) stage_load_lds<T, E, 8>(v, t, tpp, m, p, lds, L); break;
       }
-      __syncthreads();
+      lds_sync();
     }
     switch (R) {
       case 2: if constexpr (E % 2 == 0) stage_compute<T, E, 2>(v, t, tpp, Ls, ts, tw); break;
@@ -193,7 +203,7 @@ __device__ inline void stockham_from_regs(cx<T> (&v)[E], const StagePlan &P, uin
         case 5: if constexpr (E % 5 == 0) stage_store_lds<T, E, 5>(v, t, tpp, Ls, p, lds, L); break;
         default: if constexpr (E % 8 == 0) stage_store_lds<T, E, 8>(v, t, tpp, Ls, p, lds, L); break;
       }
-      __syncthreads();
+      lds_sync();
     }
   }
 }
