@@ -84,6 +84,8 @@ def main(argv=None):
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="also run a 1 GiB device-to-device copy (the reference kernel for HBM counter scaling)")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus > 1 is not supported (energies and "
@@ -111,6 +113,14 @@ def main(argv=None):
         return capi.Sim(dims, M, desc["phys_size"], desc["energies"], desc["angles"],
                         interpolation=desc["interpolation"], windowing=desc["windowing"], precision=capi.FP32,
                         pipeline=pipeline)
+
+    if a.copy_calib:
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     sim = mk(capi.PIPE_EBASIS)
     nA = sim.num_angles

@@ -120,7 +120,7 @@ __device__ inline void tensor_fields(const S *__restrict__ vox, const MatConst<T
 // A block holds pp rows of tpp threads (F::kE elements each); WG = 64 makes
 // every row exchange wave-local (the barriers compile to LDS waits).
 template <typename T, typename S, class F, int WG>
-__global__ void __launch_bounds__(WG) ebasis_polx_kernel(StagePlan P, PolxArgs<T> a, const S *__restrict__ vox,
+__global__ void __launch_bounds__(WG, F::kE <= 8 ? 4 : 2) ebasis_polx_kernel(StagePlan P, PolxArgs<T> a, const S *__restrict__ vox,
                                                          int morph, int hann) {
   constexpr int E = F::kE;
   extern __shared__ unsigned char smem_raw[];
@@ -313,6 +313,34 @@ __global__ void __launch_bounds__(256) ebasis_angles_kernel(AnglesArgs<T> a) {
   const EwaldGeom &g = a.g;
   const uint64_t P2 = (uint64_t)g.nx * g.ny;
   const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  // the block holding the pixel of FFT bin (0, 0) (X = Nx/2, Y = Ny/2) first sums
+  // the per-slice DC-neighbour partials (fixed-order tree, deterministic)
+  __shared__ double dcs[2 * kEbFields][256];
+  __shared__ cx<T> dcv[kEbFields];
+  const uint64_t t_dc = (uint64_t)(g.ny / 2) * g.nx + g.nx / 2;
+  if (blockIdx.x == t_dc / blockDim.x) {
+    for (int c = 0; c < kEbFields; c++) {
+      double sr = 0, si = 0;
+      for (uint32_t z = threadIdx.x; z < g.nz; z += blockDim.x) {
+        const double2 v = a.dcz[(uint64_t)c * g.nz + z];
+        sr += v.x;
+        si += v.y;
+      }
+      dcs[2 * c][threadIdx.x] = sr;
+      dcs[2 * c + 1][threadIdx.x] = si;
+    }
+    __syncthreads();
+    for (uint32_t w = blockDim.x / 2; w > 0; w >>= 1) {
+      if (threadIdx.x < w)
+        for (int j = 0; j < 2 * kEbFields; j++) dcs[j][threadIdx.x] += dcs[j][threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      const double cnt = g.enable2d ? 4.0 : 6.0;
+      for (int c = 0; c < kEbFields; c++) dcv[c] = mk<T>((T)(dcs[2 * c][0] / cnt), (T)(dcs[2 * c + 1][0] / cnt));
+    }
+    __syncthreads();
+  }
   if (t >= P2) return;
   const uint32_t Y = (uint32_t)(t / g.nx), X = (uint32_t)(t - (uint64_t)Y * g.nx);
   uint32_t z0 = 0;
@@ -339,27 +367,19 @@ __global__ void __launch_bounds__(256) ebasis_angles_kernel(AnglesArgs<T> a) {
       F[q][c] = s;
     }
   if (ix == 0 && iy == 0) {
-    const double cnt = three ? 6.0 : 4.0;
 #pragma unroll
     for (int q = 0; q < NG; q++) {
       const uint32_t kzq = three ? unshift(z0 + q, g.nz) : 0;
       if (kzq != 0) continue;
 #pragma unroll
-      for (int c = 0; c < kEbFields; c++) {
-        double sr = 0, si = 0;
-        for (uint32_t z = 0; z < g.nz; z++) {
-          const double2 v = a.dcz[(uint64_t)c * g.nz + z];
-          sr += v.x;
-          si += v.y;
-        }
-        F[q][c] = mk<T>((T)(sr / cnt), (T)(si / cnt));
-      }
+      for (int c = 0; c < kEbFields; c++) F[q][c] = dcv[c];
     }
   }
   double md[NG][6];
 #pragma unroll
   for (int q = 0; q < NG; q++) scatter_matrix(g, X, Y, three ? z0 + q : 0, md[q]);
   const T w1 = (T)f;
+#pragma unroll 4
   for (uint32_t an = 0; an < a.nA; an++) {
     const T *A = a.amat + (uint64_t)kEbAmat * an;
     T I[NG];
@@ -602,7 +622,7 @@ cyr_status launch_ebasis_polx(PolxArgs<T> a, const void *vox, int storage, int m
   const uint32_t wg = ct ? (a.tpp <= 64 ? 64u : a.tpp) : 256u;
   a.pp = wg / a.tpp;
   if (a.pp < 1) a.pp = 1;
-  const size_t lds = (size_t)a.pp * a.nx * sizeof(cx<T>);
+  const size_t lds = (size_t)a.pp * (ct ? ct_row_pitch(a.nx) : a.nx) * sizeof(cx<T>);
   if (lds > device_max_lds()) return fail(CYR_E_UNSUPPORTED, "E-basis: row does not fit in LDS");
   cyr_status s;
   a.twx = twiddles<T>(a.nx, &s);

@@ -101,18 +101,24 @@ __device__ inline bool bilinear_sample(const T *src, uint32_t nx, uint32_t ny, c
 #pragma clang fp contract(off)
   const double sx = inv[0] * X + inv[1] * Y + inv[2];
   const double sy = inv[3] * X + inv[4] * Y + inv[5];
-  if (!(sx >= 0 && sx <= (double)(nx - 1) && sy >= 0 && sy <= (double)(ny - 1))) return false;
-  const double fx0 = floor(sx), fy0 = floor(sy);
+  const bool inside = sx >= 0 && sx <= (double)(nx - 1) && sy >= 0 && sy <= (double)(ny - 1);
+  // outside points sample (0, 0) and are reported invalid: no branch around the loads
+  const double fx0 = inside ? floor(sx) : 0.0, fy0 = inside ? floor(sy) : 0.0;
   const uint32_t x0 = (uint32_t)fx0, y0 = (uint32_t)fy0;
-  const double fx = sx - fx0, fy = sy - fy0;
+  const double fx = inside ? sx - fx0 : 0.0, fy = inside ? sy - fy0 : 0.0;
   const double w00 = (1 - fx) * (1 - fy), w10 = fx * (1 - fy), w01 = (1 - fx) * fy, w11 = fx * fy;
+  // the four taps are loaded unconditionally (clamped into the image, so several
+  // gathers can be in flight); a tap of weight 0 is then skipped, not multiplied
+  const uint32_t x1 = x0 + 1 < nx ? x0 + 1 : x0, y1 = y0 + 1 < ny ? y0 + 1 : y0;
+  const double a00 = (double)src[(uint64_t)y0 * nx + x0], a10 = (double)src[(uint64_t)y0 * nx + x1];
+  const double a01 = (double)src[(uint64_t)y1 * nx + x0], a11 = (double)src[(uint64_t)y1 * nx + x1];
   double v = 0;
-  if (w00 != 0) v += w00 * (double)src[(uint64_t)y0 * nx + x0];
-  if (w10 != 0) v += w10 * (double)src[(uint64_t)y0 * nx + x0 + 1];
-  if (w01 != 0) v += w01 * (double)src[(uint64_t)(y0 + 1) * nx + x0];
-  if (w11 != 0) v += w11 * (double)src[(uint64_t)(y0 + 1) * nx + x0 + 1];
+  v = w00 != 0 ? v + w00 * a00 : v;
+  v = w10 != 0 ? v + w10 * a10 : v;
+  v = w01 != 0 ? v + w01 * a01 : v;
+  v = w11 != 0 ? v + w11 * a11 : v;
   *out = v;
-  return true;
+  return inside;
 }
 
 }  // namespace cyr

@@ -60,6 +60,9 @@ __global__ void __launch_bounds__(256) rotate_all_kernel(const T *__restrict__ p
   const uint32_t Y = (uint32_t)(t / nx), X = (uint32_t)(t - (uint64_t)Y * nx);
   double acc = 0;
   uint32_t cnt = 0;
+  // independent gathers per angle; unrolling lets several be in flight (the sum
+  // itself stays in angle order)
+#pragma unroll 4
   for (uint32_t a = 0; a < nA; a++) {
     double inv[6];
 #pragma unroll
