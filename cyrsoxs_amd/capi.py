@@ -104,6 +104,9 @@ _SIGS = {
     "cyr_release_workspace": (ctypes.c_int, []),
     "cyr_version": (ctypes.c_char_p, []),
     "cyr_kernel_timing": (ctypes.c_int, [ctypes.c_int]),
+    "cyr_set_tuning": (ctypes.c_int, [ctypes.POINTER(ctypes.c_uint32), ctypes.c_int32, ctypes.c_void_p]),
+    "cyr_get_tuning": (ctypes.c_int, [ctypes.POINTER(ctypes.c_uint32), ctypes.c_int32, ctypes.c_void_p]),
+    "cyr_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "cyr_kernel_timing_report": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t]),
 }
 
@@ -218,6 +221,31 @@ def simulate_device(sim: Sim, d_voxels, mats, d_out, stream=None):
     m = np.ascontiguousarray(mats, dtype=np.float64)
     check(lib().cyr_simulate_device(sim.ref, ctypes.c_void_p(d_voxels.data_ptr()), _ptr(m),
                                     ctypes.c_void_p(d_out.data_ptr()), _stream_ptr(stream)), "cyr_simulate_device")
+
+
+class YzTuning(ctypes.Structure):
+    _fields_ = [("columns", ctypes.c_uint32), ("target_blocks", ctypes.c_uint32)]
+
+
+def autotune(sim: Sim):
+    """Times the E-basis Y/z launch shapes for sim's grid and records the fastest;
+    returns (columns, target_blocks) ((0, 0): nothing to tune)."""
+    t = YzTuning()
+    check(lib().cyr_autotune(sim.ref, ctypes.byref(t)), "cyr_autotune")
+    return int(t.columns), int(t.target_blocks)
+
+
+def set_tuning(dims, precision, columns, target_blocks):
+    d = (ctypes.c_uint32 * 3)(*[int(x) for x in dims])
+    t = YzTuning(int(columns), int(target_blocks))
+    check(lib().cyr_set_tuning(d, int(precision), ctypes.byref(t)), "cyr_set_tuning")
+
+
+def get_tuning(dims, precision):
+    d = (ctypes.c_uint32 * 3)(*[int(x) for x in dims])
+    t = YzTuning()
+    check(lib().cyr_get_tuning(d, int(precision), ctypes.byref(t)), "cyr_get_tuning")
+    return int(t.columns), int(t.target_blocks)
 
 
 def polarization(sim: Sim, voxels, mats, energy_id, eangle_rad):

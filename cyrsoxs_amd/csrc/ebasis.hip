@@ -33,6 +33,10 @@
 // HBM traffic per (energy, k): morphology V (16 M) + Gx write 40 V + Gx read 40 V.
 #include <hip/hip_runtime.h>
 
+#include <map>
+#include <mutex>
+#include <tuple>
+
 #include "fft.hpp"
 #include "geom.hpp"
 #include "kernels.hpp"
@@ -413,6 +417,10 @@ template <typename T> constexpr bool e_allowed(int E) { return sizeof(T) == 4 ||
 constexpr int ct_polx_wg(int N, int E) { return N / E <= 64 ? 64 : N / E; }
 constexpr int ct_yz_pp(int N) { return N <= 512 ? 16 : (N == 1024 ? 8 : 4); }
 uint32_t ct_yz_pp_of(uint32_t n) { return n <= 512 ? 16u : (n == 1024 ? 8u : 4u); }
+// Column-tile widths compiled for the compile-time Y plans (PP x tpp <= 1024).
+bool ct_yz_pp_ok(uint32_t n, uint32_t pp) {
+  return (pp == 4 || pp == 8 || pp == 16) && pp * (n / ct_elements(n)) <= 1024;
+}
 
 template <typename T, typename S, int E>
 hipError_t launch_polx_rt(dim3 grid, dim3 block, size_t lds, hipStream_t st, const StagePlan &P,
@@ -509,13 +517,24 @@ hipError_t launch_yz_rt_e(uint32_t E, dim3 grid, dim3 block, size_t lds, hipStre
   return hipSuccess;
 }
 
-template <typename T, int NG, bool THREE, int N>
-hipError_t launch_yz_ct(dim3 grid, size_t lds, hipStream_t st, const StagePlan &P, const YzArgs<T> &a) {
-  constexpr int PP = ct_yz_pp(N);
+template <typename T, int NG, bool THREE, int N, int PP>
+hipError_t launch_yz_ct_pp(dim3 grid, size_t lds, hipStream_t st, const StagePlan &P, const YzArgs<T> &a) {
   using F = typename CtPlan<T, N, false, PP>::type;
   constexpr int WG = PP * F::TPP;
-  hipLaunchKernelGGL((ebasis_yz_kernel<T, F, NG, THREE, WG>), grid, dim3(WG), lds, st, P, a);
-  return hipGetLastError();
+  if constexpr (WG <= 1024) {
+    hipLaunchKernelGGL((ebasis_yz_kernel<T, F, NG, THREE, WG>), grid, dim3(WG), lds, st, P, a);
+    return hipGetLastError();
+  }
+  return hipErrorInvalidValue;
+}
+
+template <typename T, int NG, bool THREE, int N>
+hipError_t launch_yz_ct(dim3 grid, size_t lds, hipStream_t st, const StagePlan &P, const YzArgs<T> &a) {
+  switch (a.pp) {
+    case 4: return launch_yz_ct_pp<T, NG, THREE, N, 4>(grid, lds, st, P, a);
+    case 8: return launch_yz_ct_pp<T, NG, THREE, N, 8>(grid, lds, st, P, a);
+    default: return launch_yz_ct_pp<T, NG, THREE, N, 16>(grid, lds, st, P, a);
+  }
 }
 
 template <typename T, int NG, bool THREE>
@@ -573,6 +592,23 @@ bool ct_usable(uint32_t n, int precision) {
 
 }  // namespace
 
+namespace {
+std::mutex g_tune_mu;
+std::map<std::tuple<uint32_t, uint32_t, uint32_t, int>, YzTuning> g_tune;
+}  // namespace
+
+YzTuning ebasis_yz_tuning(uint32_t nx, uint32_t ny, uint32_t nz, int precision) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  auto it = g_tune.find(std::make_tuple(nx, ny, nz, precision));
+  return it == g_tune.end() ? YzTuning{} : it->second;
+}
+
+void ebasis_set_yz_tuning(uint32_t nx, uint32_t ny, uint32_t nz, int precision, YzTuning t) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  if (t.columns == 0 && t.target_blocks == 0) g_tune.erase(std::make_tuple(nx, ny, nz, precision));
+  else g_tune[std::make_tuple(nx, ny, nz, precision)] = t;
+}
+
 bool ebasis_supported(uint32_t nx, uint32_t ny, int precision) {
   StagePlan P;
   uint32_t lcm;
@@ -594,10 +630,11 @@ bool ebasis_supported(uint32_t nx, uint32_t ny, int precision) {
   return (size_t)ny * cs + 4096 * cs <= lds;
 }
 
-uint32_t ebasis_max_chunks(uint32_t nx, uint32_t nz) {
-  // ebasis_yz_plan starts from 16 columns per block and only lowers it (fewer
-  // blocks per slice, so fewer chunks), which makes pp = 16 the upper bound
-  return yz_chunks(nx, nz, 16, kYzTargetBlocks);
+uint32_t ebasis_max_chunks(uint32_t nx, uint32_t ny, uint32_t nz, int precision) {
+  // at most 16 columns per block; wider tiles give fewer groups, so more chunks
+  const YzTuning t = ebasis_yz_tuning(nx, ny, nz, precision);
+  const uint32_t target = t.target_blocks > kYzTargetBlocks ? t.target_blocks : kYzTargetBlocks;
+  return yz_chunks(nx, nz, 16, target);
 }
 
 cyr_status launch_ebasis_bins(const EwaldGeom &g, int32_t *kzt, hipStream_t st) {
@@ -655,15 +692,23 @@ template <typename T> cyr_status ebasis_yz_plan(YzArgs<T> &a, uint32_t target_bl
   const int prec = sizeof(T) == 8 ? CYR_FP64 : CYR_FP32;
   const size_t tw = a.nz > 1 ? (size_t)a.nz * sizeof(cx<T>) : 0;
   auto lds_of = [&](uint32_t q) { return (size_t)q * a.ny * sizeof(cx<T>) + tw; };
+  // explicit arguments win over the tuning table, which wins over the defaults
+  const YzTuning tune = ebasis_yz_tuning(a.nx, a.ny, a.nz, prec);
+  if (!target_blocks) target_blocks = tune.target_blocks ? tune.target_blocks : kYzTargetBlocks;
+  const uint32_t want_pp = a.pp ? a.pp : tune.columns;
   a.ct = 0;
-  if (!a.e && !a.pp && ct_usable(a.ny, prec) && lds_of(ct_yz_pp_of(a.ny)) <= device_max_lds()) {
-    a.ct = 1;
-    a.e = ct_elements(a.ny);
-    a.pp = ct_yz_pp_of(a.ny);
-    a.tpp = a.ny / a.e;
-  } else {
+  if (!a.e && ct_usable(a.ny, prec)) {
+    const uint32_t pp = ct_yz_pp_ok(a.ny, want_pp) ? want_pp : ct_yz_pp_of(a.ny);
+    if (lds_of(pp) <= device_max_lds()) {
+      a.ct = 1;
+      a.e = ct_elements(a.ny);
+      a.pp = pp;
+      a.tpp = a.ny / a.e;
+    }
+  }
+  if (!a.ct) {
     const uint32_t ecap = sizeof(T) == 8 ? 8 : 16;
-    uint32_t pp = a.pp ? a.pp : 16;
+    uint32_t pp = want_pp ? want_pp : 16;
     uint32_t E = a.e ? a.e : choose_e(a.ny, lcm, yz_wg_of(lcm) / pp, ecap);
     if (E % lcm != 0 || a.ny % E != 0 || !e_instantiated(E, prec)) E = lcm;
     const uint32_t tpp = a.ny / E;
@@ -686,7 +731,7 @@ template <typename T> cyr_status launch_ebasis_yz(YzArgs<T> a, int ng, hipStream
       a.tpp != a.ny / a.e)
     return fail(CYR_E_INVALID, "E-basis: Y plan missing (call ebasis_yz_plan)");
   const int prec = sizeof(T) == 8 ? CYR_FP64 : CYR_FP32;
-  if (a.ct && (!ct_usable(a.ny, prec) || a.e != ct_elements(a.ny) || a.pp != ct_yz_pp_of(a.ny)))
+  if (a.ct && (!ct_usable(a.ny, prec) || a.e != ct_elements(a.ny) || !ct_yz_pp_ok(a.ny, a.pp)))
     return fail(CYR_E_INVALID, "E-basis: inconsistent compile-time Y plan");
   if (!a.ct && a.pp * a.tpp > yz_wg_of(a.e)) return fail(CYR_E_INVALID, "E-basis: Y block larger than its launch bound");
   const bool three = a.nz > 1;

@@ -243,7 +243,7 @@ size_t max_lds_bytes() {
     hipGetDevice(&dev);
     int b = 0;
     if (hipDeviceGetAttribute(&b, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || b <= 0)
-      b = 65536;
+      b = 160 * 1024;  // gfx950 (the only target this library is built for)
     v = (size_t)b;
   }
   return v;

@@ -70,7 +70,7 @@ bool ebasis_supported(uint32_t nx, uint32_t ny, int precision);
 // Grid size the Y/z kernel's z chunking aims for, and the most chunks any plan
 // for (nx, nz) uses (sizes the partial-sum buffer).
 constexpr uint32_t kYzTargetBlocks = 1024;
-uint32_t ebasis_max_chunks(uint32_t nx, uint32_t nz);
+uint32_t ebasis_max_chunks(uint32_t nx, uint32_t ny, uint32_t nz, int precision);
 // Pixel -> FFT z bin of its (kx, ky) column: kzt[ky][kx] (-1 = not sampled); with
 // linear interpolation the upper bin is kzt - 1 (mod Nz).
 cyr_status launch_ebasis_bins(const EwaldGeom &g, int32_t *kzt, hipStream_t st);
@@ -99,8 +99,17 @@ template <typename T> struct YzArgs {
   uint32_t e, pp, tpp, zchunk, nchunks;  // 0 = automatic (ebasis_yz_plan fills)
   uint32_t ct;                     // 1: compile-time FFT plan for Ny (set by ebasis_yz_plan)
 };
+// Launch choices of the Y/z kernel for one (Nx, Ny, Nz, precision) shape, set by
+// the autotuner: columns per block and the grid size the z chunking aims for
+// (0 = built-in default).
+struct YzTuning {
+  uint32_t columns = 0;
+  uint32_t target_blocks = 0;
+};
+YzTuning ebasis_yz_tuning(uint32_t nx, uint32_t ny, uint32_t nz, int precision);
+void ebasis_set_yz_tuning(uint32_t nx, uint32_t ny, uint32_t nz, int precision, YzTuning t);
 // Fills e, pp, tpp, zchunk, nchunks (fields left 0 are chosen automatically;
-// target_blocks is the grid size the z chunking aims for).
+// target_blocks is the grid size the z chunking aims for, 0 = tuned/default).
 template <typename T> cyr_status ebasis_yz_plan(YzArgs<T> &a, uint32_t target_blocks);
 template <typename T> cyr_status launch_ebasis_yz(YzArgs<T> a, int ng, hipStream_t st);
 

@@ -215,7 +215,7 @@ cyr_status run_impl(const cyr_sim_desc *d, const SimGeometry &g, const void *d_v
     yz.nx = g.nx;
     yz.ny = g.ny;
     yz.nz = g.nz;
-    CYR_TRY(ebasis_yz_plan<T>(yz, kYzTargetBlocks));
+    CYR_TRY(ebasis_yz_plan<T>(yz, 0));
     CYR_TRY(ensure_buffer(wb.bins, sizeof(int32_t) * P));
     CYR_TRY(ensure_buffer(wb.part, sizeof(cx<T>) * P * ng * kEbFields * yz.nchunks));
     CYR_TRY(ensure_buffer(wb.dc, sizeof(double2) * kEbFields * g.nz));
@@ -416,7 +416,7 @@ uint64_t work_bytes(const cyr_sim_desc *d, const SimGeometry &g) {
                       !(d->precision == CYR_FP64 && d->storage != CYR_STORE_NATIVE);
   uint64_t b = cs * g.V * (ebasis ? kEbFields : 3);
   if (ebasis) {  // pixel -> bin table, Y/z partial sums (chunks x 2 bins x fields), DC probes
-    const uint64_t nch = ebasis_max_chunks(g.nx, g.nz);
+    const uint64_t nch = ebasis_max_chunks(g.nx, g.ny, g.nz, d->precision);
     b += 4 * g.P + cs * g.P * 2 * kEbFields * nch + 16 * kEbFields * g.nz;
   }
   if (!ebasis && d->algorithm == 1) b += cs * g.V * 6;
@@ -453,11 +453,118 @@ void convert_storage(const cyr_sim_desc *d, const void *host_vox, std::vector<un
   }
 }
 
+// Autotuner of the E-basis Y/z kernel: times every candidate launch shape
+// (columns per work group x target grid size) on the descriptor's grid with
+// zero-filled fields and the first energy's Ewald bins, and records the fastest
+// in the tuning table.  The choice does not change results beyond the order of
+// the partial sums over z chunks.
+template <typename T>
+cyr_status autotune_impl(const cyr_sim_desc *d, const SimGeometry &g, WorkBuffers &wb, hipStream_t st,
+                         YzTuning *chosen) {
+  *chosen = YzTuning{};
+  const bool ebasis = ebasis_supported(g.nx, g.ny, d->precision) &&
+                      !(d->precision == CYR_FP64 && d->storage != CYR_STORE_NATIVE);
+  if (!ebasis || g.nz < 2) return CYR_OK;  // 2D: one slice, nothing to split
+  const uint64_t V = g.V, P = g.P;
+  const int ng = d->interpolation == 1 ? 2 : 1;
+  CYR_TRY(ensure_buffer(wb.pol, sizeof(cx<T>) * V * kEbFields));
+  CYR_HIP(hipMemsetAsync(wb.pol.ptr, 0, sizeof(cx<T>) * V * kEbFields, st));
+  CYR_TRY(ensure_buffer(wb.bins, sizeof(int32_t) * P));
+  const double kv[3] = {g.kvec[0].x, g.kvec[0].y, g.kvec[0].z};
+  const EwaldGeom geo = make_ewald_geom(g.nx, g.ny, g.nz, d->phys_size, g.kmag[0], kv, d->interpolation);
+  CYR_TRY(launch_ebasis_bins(geo, reinterpret_cast<int32_t *>(wb.bins.ptr), st));
+  cyr_status s;
+  const cx<T> *twz = twiddles<T>(g.nz, &s);
+  if (!twz) return s;
+  hipEvent_t e0, e1;
+  CYR_HIP(hipEventCreate(&e0));
+  CYR_HIP(hipEventCreate(&e1));
+  float best = -1;
+  YzTuning pick{};
+  const uint32_t cols[3] = {4, 8, 16};
+  const uint32_t targets[4] = {256, 512, 1024, 2048};
+  cyr_status rc = CYR_OK;
+  for (uint32_t ci = 0; ci < 3 && rc == CYR_OK; ci++)
+    for (uint32_t ti = 0; ti < 4 && rc == CYR_OK; ti++) {
+      YzArgs<T> ya{};
+      ya.nx = g.nx;
+      ya.ny = g.ny;
+      ya.nz = g.nz;
+      ya.pp = cols[ci];
+      if (ebasis_yz_plan<T>(ya, targets[ti]) != CYR_OK || ya.pp != cols[ci]) continue;
+      rc = ensure_buffer(wb.part, sizeof(cx<T>) * P * ng * kEbFields * ya.nchunks);
+      if (rc != CYR_OK) break;
+      for (int c = 0; c < kEbFields; c++) ya.vol[c] = reinterpret_cast<cx<T> *>(wb.pol.ptr) + (uint64_t)c * V;
+      ya.part = reinterpret_cast<cx<T> *>(wb.part.ptr);
+      ya.kzt = reinterpret_cast<int32_t *>(wb.bins.ptr);
+      ya.twz = twz;
+      rc = launch_ebasis_yz<T>(ya, ng, st);  // warm-up
+      if (rc != CYR_OK) break;
+      constexpr int reps = 3;
+      if (hipEventRecord(e0, st) != hipSuccess) { rc = fail(CYR_E_DEVICE, "event record"); break; }
+      for (int r = 0; r < reps && rc == CYR_OK; r++) rc = launch_ebasis_yz<T>(ya, ng, st);
+      if (rc != CYR_OK) break;
+      float ms = 0;
+      if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+          hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+        rc = fail(CYR_E_DEVICE, "autotune timing");
+        break;
+      }
+      if (best < 0 || ms < best) {
+        best = ms;
+        pick = YzTuning{cols[ci], targets[ti]};
+      }
+    }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc != CYR_OK) return rc;
+  if (best >= 0) ebasis_set_yz_tuning(g.nx, g.ny, g.nz, d->precision, pick);
+  *chosen = pick;
+  return CYR_OK;
+}
+
 }  // namespace cyr
+
 
 using namespace cyr;
 
 extern "C" {
+
+cyr_status cyr_set_tuning(const uint32_t dims[3], int32_t precision, const cyr_yz_tuning *t) {
+  if (!dims || !t) return fail(CYR_E_INVALID, "null argument");
+  if (t->columns != 0 && t->columns != 1 && t->columns != 2 && t->columns != 4 && t->columns != 8 &&
+      t->columns != 16)
+    return fail(CYR_E_INVALID, "columns must be 0 or a power of two <= 16");
+  ebasis_set_yz_tuning(dims[0], dims[1], dims[2], precision, YzTuning{t->columns, t->target_blocks});
+  return CYR_OK;
+}
+
+cyr_status cyr_get_tuning(const uint32_t dims[3], int32_t precision, cyr_yz_tuning *t) {
+  if (!dims || !t) return fail(CYR_E_INVALID, "null argument");
+  const YzTuning y = ebasis_yz_tuning(dims[0], dims[1], dims[2], precision);
+  t->columns = y.columns;
+  t->target_blocks = y.target_blocks;
+  return CYR_OK;
+}
+
+cyr_status cyr_autotune(const cyr_sim_desc *desc, cyr_yz_tuning *chosen) {
+  SimGeometry g;
+  CYR_TRY(build_geometry(desc, g));
+  DeviceState *ds;
+  CYR_TRY(device_state(&ds));
+  std::lock_guard<std::mutex> lk(ds->mu);
+  YzTuning y{};
+  const cyr_status s = desc->precision == CYR_FP64 ? autotune_impl<double>(desc, g, ds->wb, ds->stream, &y)
+                                                   : autotune_impl<float>(desc, g, ds->wb, ds->stream, &y);
+  if (s != CYR_OK) return s;
+  if (hipStreamSynchronize(ds->stream) != hipSuccess) return fail(CYR_E_DEVICE, "autotune sync");
+  if (chosen) {
+    chosen->columns = y.columns;
+    chosen->target_blocks = y.target_blocks;
+  }
+  return CYR_OK;
+}
+
 
 uint64_t cyr_footprint_bytes(const cyr_sim_desc *desc) {
   SimGeometry g;

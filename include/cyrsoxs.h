@@ -178,6 +178,25 @@ cyr_status cyr_engine_cancel(cyr_engine eng, uint64_t request_id);
 cyr_status cyr_engine_get_stats(cyr_engine eng, cyr_engine_stats *stats);
 cyr_status cyr_engine_destroy(cyr_engine eng);
 
+/* ---- autotuning ------------------------------------------------------------ */
+
+/* Launch choices of the E-basis executor's Y-FFT / z-bin kernel for one grid
+ * shape (Nx, Ny, Nz) and precision: columns per work group (0 or a power of two
+ * <= 16) and the grid size its split of z aims for (0 = built-in defaults).
+ * No counterpart in the reference (cuFFT plans its own launches). */
+typedef struct {
+  uint32_t columns;
+  uint32_t target_blocks;
+} cyr_yz_tuning;
+
+/* Records / reads the process-wide choice for a shape ({0, 0} clears it). */
+cyr_status cyr_set_tuning(const uint32_t dims[3], int32_t precision, const cyr_yz_tuning *t);
+cyr_status cyr_get_tuning(const uint32_t dims[3], int32_t precision, cyr_yz_tuning *t);
+/* Times the candidate launch shapes for desc's grid on the current device (HIP
+ * events, zero-filled fields) and records the fastest; *chosen (may be NULL)
+ * receives it ({0, 0} when the grid has nothing to tune, e.g. 2D). */
+cyr_status cyr_autotune(const cyr_sim_desc *desc, cyr_yz_tuning *chosen);
+
 /* ---- utilities ------------------------------------------------------------- */
 
 const char *cyr_status_string(cyr_status s);

@@ -107,7 +107,7 @@ def test_footprint_accounts_for_pipeline():
     V = 512 * 512 * 128
     fp, fe = plain.footprint(), eb.footprint()
     assert fp >= V * (16 * 4 + 24)  # morphology + 3 complex64 volumes
-    assert fe >= V * (16 * 4 + 48)  # morphology + 6 complex64 volumes
+    assert fe >= V * (16 * 4 + 40)  # morphology + 5 complex64 tensor fields
     assert fe > fp
     half = _sim(dims=(512, 512, 128), num_materials=4, storage=capi.STORE_FP16)
     assert plain.footprint() - half.footprint() == V * 4 * 8
