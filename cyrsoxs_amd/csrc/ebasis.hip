@@ -121,61 +121,66 @@ __device__ inline void tensor_fields(const S *__restrict__ vox, const MatConst<T
 }
 
 // ---- tensor fields + X FFT ------------------------------------------------------
-// A block holds pp rows of tpp threads (F::kE elements each); WG = 64 makes
-// every row exchange wave-local (the barriers compile to LDS waits).
+// A block holds pp rows of tpp threads (F::kE elements each) and walks rpb such
+// row groups; WG = 64 makes every row exchange wave-local (the barriers compile
+// to LDS waits).  LDS: the pp x row tile.  The X twiddles are read from the
+// global table: with many independent single-wave blocks per CU their latency
+// hides, while staging them per block cost more than it saved.
 template <typename T, typename S, class F, int WG>
-__global__ void __launch_bounds__(WG, F::kE <= 8 ? 4 : 2) ebasis_polx_kernel(StagePlan P, PolxArgs<T> a, const S *__restrict__ vox,
-                                                         int morph, int hann) {
+__global__ void __launch_bounds__(WG, F::kE <= 8 ? 4 : 2)
+    ebasis_polx_kernel(StagePlan P, PolxArgs<T> a, const S *__restrict__ vox, int morph, int hann) {
   constexpr int E = F::kE;
   extern __shared__ unsigned char smem_raw[];
   cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
   const F fft(P, a.tpp, a.nx, true);
   const uint32_t tpp = a.tpp;
   const uint32_t t = threadIdx.x % tpp, p = threadIdx.x / tpp;
-  const uint32_t row = blockIdx.x * a.pp + p;  // row = y + ny z
-  const bool active = row < a.rows;
-  const uint32_t y = active ? row % a.ny : 0, z = active ? row / a.ny : 0;
   const T k4 = (T)(1.0 / (4.0 * M_PI));
-  const T wyz = hann ? a.hy[y] * a.hz[z] * k4 : k4;
   const bool rot = a.rot != 0;
-  T fre[kEbFields][E], fim[kEbFields][E];
+  for (uint32_t rb = 0; rb < a.rpb; rb++) {
+    const uint32_t row = (blockIdx.x * a.rpb + rb) * a.pp + p;  // row = y + ny z
+    const bool active = row < a.rows;
+    const uint32_t y = active ? row % a.ny : 0, z = active ? row / a.ny : 0;
+    const T wyz = hann ? a.hy[y] * a.hz[z] * k4 : k4;
+    T fre[kEbFields][E], fim[kEbFields][E];
 #pragma unroll
-  for (int i = 0; i < E; i++) {
-    const uint32_t x = fft.in_pos(t, i);
-    T fr[kEbFields], fi[kEbFields];
-    if (active) {
-      const uint64_t idx = (uint64_t)row * a.nx + x;
-      if (morph == 1) tensor_fields<T, S, 1>(vox, a.mats, a.M, a.V, idx, a.Q, rot, fr, fi);
-      else tensor_fields<T, S, 0>(vox, a.mats, a.M, a.V, idx, a.Q, rot, fr, fi);
-      const T w = hann ? a.hx[x] * wyz : wyz;
+    for (int i = 0; i < E; i++) {
+      const uint32_t x = fft.in_pos(t, i);
+      T fr[kEbFields], fi[kEbFields];
+      if (active) {
+        const uint64_t idx = (uint64_t)row * a.nx + x;
+        if (morph == 1) tensor_fields<T, S, 1>(vox, a.mats, a.M, a.V, idx, a.Q, rot, fr, fi);
+        else tensor_fields<T, S, 0>(vox, a.mats, a.M, a.V, idx, a.Q, rot, fr, fi);
+        const T w = hann ? a.hx[x] * wyz : wyz;
+#pragma unroll
+        for (int c = 0; c < kEbFields; c++) {
+          fr[c] *= w;
+          fi[c] *= w;
+        }
+      } else {
+#pragma unroll
+        for (int c = 0; c < kEbFields; c++) fr[c] = fi[c] = 0;
+      }
 #pragma unroll
       for (int c = 0; c < kEbFields; c++) {
-        fr[c] *= w;
-        fi[c] *= w;
+        fre[c][i] = fr[c];
+        fim[c][i] = fi[c];
       }
-    } else {
-#pragma unroll
-      for (int c = 0; c < kEbFields; c++) fr[c] = fi[c] = 0;
     }
 #pragma unroll
     for (int c = 0; c < kEbFields; c++) {
-      fre[c][i] = fr[c];
-      fim[c][i] = fi[c];
+      cx<T> v[E];
+#pragma unroll
+      for (int i = 0; i < E; i++) v[i] = mk<T>(fre[c][i], fim[c][i]);
+      fft.run(v, t, p, lds, a.twx);
+      if (active) {
+        cx<T> *o = a.out[c] + (uint64_t)row * a.nx;
+#pragma unroll
+        for (int i = 0; i < E; i++) o[fft.out_pos(t, i)] = v[i];
+      }
+      // (run() ends with a barrier after its last LDS reads, so the next field's
+      // or row's first LDS store cannot overtake them)
     }
-  }
-#pragma unroll
-  for (int c = 0; c < kEbFields; c++) {
-    cx<T> v[E];
-#pragma unroll
-    for (int i = 0; i < E; i++) v[i] = mk<T>(fre[c][i], fim[c][i]);
-    fft.run(v, t, p, lds, a.twx);
-    if (active) {
-      cx<T> *o = a.out[c] + (uint64_t)row * a.nx;
-#pragma unroll
-      for (int i = 0; i < E; i++) o[fft.out_pos(t, i)] = v[i];
-    }
-    // (run() ends with a barrier after its last LDS reads, so the next field's
-    // first LDS store cannot overtake them)
   }
 }
 
@@ -210,7 +215,11 @@ __global__ void __launch_bounds__(WG) ebasis_yz_kernel(StagePlan P, YzArgs<T> a)
     }
     return;
   } else {
-    cx<T> *ltw = lds + (size_t)pp * a.ny;  // z twiddles W^k, k < nz
+    // LDS: [pp x ny tile][ny Y twiddles][nz z twiddles]; the Y twiddles are
+    // reread every slice, so they are staged next to the tile
+    cx<T> *lty = lds + (size_t)pp * a.ny;
+    cx<T> *ltw = lty + a.ny;
+    for (uint32_t i = threadIdx.x; i < a.ny; i += blockDim.x) lty[i] = a.twy[i];
     for (uint32_t i = threadIdx.x; i < a.nz; i += blockDim.x) ltw[i] = a.twz[i];
     const uint32_t zb = blockIdx.z * a.zchunk;
     const uint32_t ze = zb + a.zchunk < a.nz ? zb + a.zchunk : a.nz;
@@ -240,7 +249,7 @@ __global__ void __launch_bounds__(WG) ebasis_yz_kernel(StagePlan P, YzArgs<T> a)
 #pragma unroll
       for (int i = 0; i < E; i++) v[i] = vn[i];
       if (z + 1 < ze) load(z + 1, vn);
-      fft.run(v, t, p, lds, a.twy);
+      fft.run(v, t, p, lds, lty);
       cx<T> wz = mk<T>(1, 0);  // W^(-z): second bin of the linear pair
       if constexpr (NG == 2) wz = ltw[z == 0 ? 0 : a.nz - z];
 #pragma unroll
@@ -609,25 +618,27 @@ void ebasis_set_yz_tuning(uint32_t nx, uint32_t ny, uint32_t nz, int precision, 
   else g_tune[std::make_tuple(nx, ny, nz, precision)] = t;
 }
 
-bool ebasis_supported(uint32_t nx, uint32_t ny, int precision) {
+bool ebasis_supported(uint32_t nx, uint32_t ny, uint32_t nz, int precision) {
   StagePlan P;
   uint32_t lcm;
   const size_t cs = precision == CYR_FP64 ? 16 : 8;
   const uint32_t ecap = precision == CYR_FP64 ? 8 : 16;
   const size_t lds = device_max_lds();
-  if (nx < 2 || ny < 2) return false;
+  if (nx < 2 || ny < 2 || nz > 0xffffu) return false;
   if (!ct_usable(nx, precision)) {  // run-time X plan: one row per <= 256 threads
     if (!make_stage_plan(nx, P, lcm)) return false;
     const uint32_t ex = choose_e(nx, lcm, 256, ecap);
-    if (!e_instantiated(ex, precision) || nx / ex > 256 || (size_t)nx * cs > lds) return false;
+    if (!e_instantiated(ex, precision) || nx / ex > 256) return false;
+    const size_t pp = 256 / (nx / ex);
+    if (pp * nx * cs > lds) return false;  // row tile
   }
-  // the Y kernel's narrowest run-time block: one column (pp = 1); the z twiddles
-  // take at most 4096 entries of LDS
+  // Y kernel LDS: tile + (3D) Y and z twiddles; the run-time plan can narrow to one column
+  const size_t tw = nz > 1 ? ((size_t)ny + nz) * cs : 0;
+  if (ct_usable(ny, precision) && (size_t)ct_yz_pp_of(ny) * ny * cs + tw <= lds) return true;
   if (!make_stage_plan(ny, P, lcm)) return false;
   const uint32_t ey = choose_e(ny, lcm, yz_wg_of(lcm), ecap);
-  if (ct_usable(ny, precision) && (size_t)ct_yz_pp_of(ny) * ny * cs + 4096 * cs <= lds) return true;
   if (!e_instantiated(ey, precision) || ny / ey > yz_wg_of(ey)) return false;
-  return (size_t)ny * cs + 4096 * cs <= lds;
+  return (size_t)ny * cs + tw <= lds;
 }
 
 uint32_t ebasis_max_chunks(uint32_t nx, uint32_t ny, uint32_t nz, int precision) {
@@ -659,12 +670,14 @@ cyr_status launch_ebasis_polx(PolxArgs<T> a, const void *vox, int storage, int m
   const uint32_t wg = ct ? (a.tpp <= 64 ? 64u : a.tpp) : 256u;
   a.pp = wg / a.tpp;
   if (a.pp < 1) a.pp = 1;
-  const size_t lds = (size_t)a.pp * (ct ? ct_row_pitch(a.nx) : a.nx) * sizeof(cx<T>);
+  a.pitch = ct ? ct_row_pitch(a.nx) : a.nx;
+  a.rpb = 1;  // row groups per block
+  const size_t lds = (size_t)a.pp * a.pitch * sizeof(cx<T>);
   if (lds > device_max_lds()) return fail(CYR_E_UNSUPPORTED, "E-basis: row does not fit in LDS");
   cyr_status s;
   a.twx = twiddles<T>(a.nx, &s);
   if (!a.twx) return s;
-  const dim3 grid(ceil_div(a.rows, a.pp)), block(a.pp * a.tpp);
+  const dim3 grid(ceil_div(a.rows, a.pp * a.rpb)), block(a.pp * a.tpp);
   bool found = false;
   hipError_t e = hipSuccess;
   auto go = [&](auto tag) {
@@ -690,7 +703,7 @@ template <typename T> cyr_status ebasis_yz_plan(YzArgs<T> &a, uint32_t target_bl
   uint32_t lcm;
   if (!make_stage_plan(a.ny, P, lcm)) return fail(CYR_E_UNSUPPORTED, "E-basis: Ny must be 2^a 3^b 5^c");
   const int prec = sizeof(T) == 8 ? CYR_FP64 : CYR_FP32;
-  const size_t tw = a.nz > 1 ? (size_t)a.nz * sizeof(cx<T>) : 0;
+  const size_t tw = a.nz > 1 ? ((size_t)a.nz + a.ny) * sizeof(cx<T>) : 0;
   auto lds_of = [&](uint32_t q) { return (size_t)q * a.ny * sizeof(cx<T>) + tw; };
   // explicit arguments win over the tuning table, which wins over the defaults
   const YzTuning tune = ebasis_yz_tuning(a.nx, a.ny, a.nz, prec);
@@ -743,7 +756,8 @@ template <typename T> cyr_status launch_ebasis_yz(YzArgs<T> a, int ng, hipStream
   if (!a.twy) return s;
   const uint32_t groups = ceil_div(a.nx, a.pp);
   const dim3 grid(groups, kEbFields, a.nchunks), block(a.pp * a.tpp);
-  const size_t lds = (size_t)a.pp * a.ny * sizeof(cx<T>) + (three ? (size_t)a.nz * sizeof(cx<T>) : 0);
+  const size_t lds =
+      (size_t)a.pp * a.ny * sizeof(cx<T>) + (three ? ((size_t)a.nz + a.ny) * sizeof(cx<T>) : 0);
   if (lds > device_max_lds()) return fail(CYR_E_UNSUPPORTED, "E-basis: Y tile does not fit in LDS");
   bool found = false;
   hipError_t e;

@@ -65,8 +65,9 @@ cyr_status launch_igor_shift(cx<T> *vol, uint32_t nx, uint32_t ny, uint32_t nz, 
 // p~ = A F.
 constexpr int kEbFields = 5;
 constexpr int kEbAmat = 3 * kEbFields;
-// True when both in-plane axis lengths have a Stockham plan that fits the fused kernels.
-bool ebasis_supported(uint32_t nx, uint32_t ny, int precision);
+// True when the grid has Stockham plans and LDS tiles that fit the fused kernels
+// (other grids run the plain executor).
+bool ebasis_supported(uint32_t nx, uint32_t ny, uint32_t nz, int precision);
 // Grid size the Y/z kernel's z chunking aims for, and the most chunks any plan
 // for (nx, nz) uses (sizes the partial-sum buffer).
 constexpr uint32_t kYzTargetBlocks = 1024;
@@ -84,7 +85,7 @@ template <typename T> struct PolxArgs {
   Mat3<T> Q;                       // columns e1, e2, k^ (row-major)
   int32_t rot;                     // 0 when Q is the identity
   const cx<T> *twx;                // filled by the launcher
-  uint32_t tpp, pp;                // filled by the launcher
+  uint32_t tpp, pp, pitch, rpb;    // filled by the launcher
   cx<T> *out[kEbFields];           // X-transformed tensor fields
 };
 template <typename T>

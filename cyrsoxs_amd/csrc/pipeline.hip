@@ -66,6 +66,14 @@ template <typename T> MatConst<T> mat_const(const double *m4) {
   return r;
 }
 
+// The fused E-basis kernels need 2^a 3^b 5^c in-plane lengths and tiles that fit
+// in LDS; FP64 runs them on native-precision morphologies only.  Other
+// simulations asking for E-basis run the plain executor.
+bool ebasis_usable(const cyr_sim_desc *d, const SimGeometry &g) {
+  return ebasis_supported(g.nx, g.ny, g.nz, d->precision) &&
+         !(d->precision == CYR_FP64 && d->storage != CYR_STORE_NATIVE);
+}
+
 // Per-simulation constant tables, uploaded once.
 struct Tables {
   size_t mats = 0, han = 0, amat = 0, rotinv = 0, detinv = 0, total = 0;
@@ -162,9 +170,7 @@ template <typename T>
 cyr_status run_impl(const cyr_sim_desc *d, const SimGeometry &g, const void *d_vox, const double *mats, T *d_out,
                     WorkBuffers &wb, hipStream_t st) {
   const uint64_t V = g.V, P = g.P;
-  // the fused E-basis kernels need 2^a 3^b 5^c in-plane lengths; others run the plain executor
-  const bool ebasis = d->pipeline == CYR_PIPE_EBASIS && ebasis_supported(g.nx, g.ny, d->precision) &&
-                      !(d->precision == CYR_FP64 && d->storage != CYR_STORE_NATIVE);
+  const bool ebasis = d->pipeline == CYR_PIPE_EBASIS && ebasis_usable(d, g);
   const bool alg1 = d->algorithm == 1;
   const bool hann = d->windowing == 1 && !alg1;  // Algorithm 1 has no window (include/uniaxial.h:441-521)
   const bool matframe = d->reference_frame == 0;
@@ -411,9 +417,7 @@ uint64_t output_bytes(const cyr_sim_desc *d, const SimGeometry &g) {
 
 uint64_t work_bytes(const cyr_sim_desc *d, const SimGeometry &g) {
   const uint64_t cs = d->precision == CYR_FP64 ? 16 : 8, rs = cs / 2;
-  // the fused E-basis kernels need 2^a 3^b 5^c in-plane lengths; others run the plain executor
-  const bool ebasis = d->pipeline == CYR_PIPE_EBASIS && ebasis_supported(g.nx, g.ny, d->precision) &&
-                      !(d->precision == CYR_FP64 && d->storage != CYR_STORE_NATIVE);
+  const bool ebasis = d->pipeline == CYR_PIPE_EBASIS && ebasis_usable(d, g);
   uint64_t b = cs * g.V * (ebasis ? kEbFields : 3);
   if (ebasis) {  // pixel -> bin table, Y/z partial sums (chunks x 2 bins x fields), DC probes
     const uint64_t nch = ebasis_max_chunks(g.nx, g.ny, g.nz, d->precision);
@@ -462,8 +466,7 @@ template <typename T>
 cyr_status autotune_impl(const cyr_sim_desc *d, const SimGeometry &g, WorkBuffers &wb, hipStream_t st,
                          YzTuning *chosen) {
   *chosen = YzTuning{};
-  const bool ebasis = ebasis_supported(g.nx, g.ny, d->precision) &&
-                      !(d->precision == CYR_FP64 && d->storage != CYR_STORE_NATIVE);
+  const bool ebasis = ebasis_usable(d, g);
   if (!ebasis || g.nz < 2) return CYR_OK;  // 2D: one slice, nothing to split
   const uint64_t V = g.V, P = g.P;
   const int ng = d->interpolation == 1 ? 2 : 1;
