@@ -107,6 +107,8 @@ _SIGS = {
     "cyr_set_tuning": (ctypes.c_int, [ctypes.POINTER(ctypes.c_uint32), ctypes.c_int32, ctypes.c_void_p]),
     "cyr_get_tuning": (ctypes.c_int, [ctypes.POINTER(ctypes.c_uint32), ctypes.c_int32, ctypes.c_void_p]),
     "cyr_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "cyr_set_energy_lanes": (ctypes.c_int, [ctypes.c_uint32]),
+    "cyr_energy_lanes": (ctypes.c_uint32, []),
     "cyr_kernel_timing_report": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t]),
 }
 
@@ -233,6 +235,14 @@ def autotune(sim: Sim):
     t = YzTuning()
     check(lib().cyr_autotune(sim.ref, ctypes.byref(t)), "cyr_autotune")
     return int(t.columns), int(t.target_blocks)
+
+
+def set_energy_lanes(lanes):
+    check(lib().cyr_set_energy_lanes(int(lanes)), "cyr_set_energy_lanes")
+
+
+def energy_lanes():
+    return int(lib().cyr_energy_lanes())
 
 
 def set_tuning(dims, precision, columns, target_blocks):

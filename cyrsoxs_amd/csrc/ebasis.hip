@@ -121,11 +121,12 @@ __device__ inline void tensor_fields(const S *__restrict__ vox, const MatConst<T
 }
 
 // ---- tensor fields + X FFT ------------------------------------------------------
-// A block holds pp rows of tpp threads (F::kE elements each) and walks rpb such
-// row groups; WG = 64 makes every row exchange wave-local (the barriers compile
-// to LDS waits).  LDS: the pp x row tile.  The X twiddles are read from the
-// global table: with many independent single-wave blocks per CU their latency
-// hides, while staging them per block cost more than it saved.
+// A block holds pp rows of tpp threads (F::kE elements each); WG = 64 makes
+// every row exchange wave-local (the barriers compile to LDS waits).  LDS: the
+// pp x row tile.  The X twiddles are read from the global table: with many
+// independent single-wave blocks per CU their latency hides, while staging them
+// per block cost more than it saved (as did looping a block over several rows,
+// which spills registers).
 template <typename T, typename S, class F, int WG>
 __global__ void __launch_bounds__(WG, F::kE <= 8 ? 4 : 2)
     ebasis_polx_kernel(StagePlan P, PolxArgs<T> a, const S *__restrict__ vox, int morph, int hann) {
@@ -135,52 +136,50 @@ __global__ void __launch_bounds__(WG, F::kE <= 8 ? 4 : 2)
   const F fft(P, a.tpp, a.nx, true);
   const uint32_t tpp = a.tpp;
   const uint32_t t = threadIdx.x % tpp, p = threadIdx.x / tpp;
+  const uint32_t row = blockIdx.x * a.pp + p;  // row = y + ny z
+  const bool active = row < a.rows;
+  const uint32_t y = active ? row % a.ny : 0, z = active ? row / a.ny : 0;
   const T k4 = (T)(1.0 / (4.0 * M_PI));
+  const T wyz = hann ? a.hy[y] * a.hz[z] * k4 : k4;
   const bool rot = a.rot != 0;
-  for (uint32_t rb = 0; rb < a.rpb; rb++) {
-    const uint32_t row = (blockIdx.x * a.rpb + rb) * a.pp + p;  // row = y + ny z
-    const bool active = row < a.rows;
-    const uint32_t y = active ? row % a.ny : 0, z = active ? row / a.ny : 0;
-    const T wyz = hann ? a.hy[y] * a.hz[z] * k4 : k4;
-    T fre[kEbFields][E], fim[kEbFields][E];
+  T fre[kEbFields][E], fim[kEbFields][E];
 #pragma unroll
-    for (int i = 0; i < E; i++) {
-      const uint32_t x = fft.in_pos(t, i);
-      T fr[kEbFields], fi[kEbFields];
-      if (active) {
-        const uint64_t idx = (uint64_t)row * a.nx + x;
-        if (morph == 1) tensor_fields<T, S, 1>(vox, a.mats, a.M, a.V, idx, a.Q, rot, fr, fi);
-        else tensor_fields<T, S, 0>(vox, a.mats, a.M, a.V, idx, a.Q, rot, fr, fi);
-        const T w = hann ? a.hx[x] * wyz : wyz;
-#pragma unroll
-        for (int c = 0; c < kEbFields; c++) {
-          fr[c] *= w;
-          fi[c] *= w;
-        }
-      } else {
-#pragma unroll
-        for (int c = 0; c < kEbFields; c++) fr[c] = fi[c] = 0;
-      }
+  for (int i = 0; i < E; i++) {
+    const uint32_t x = fft.in_pos(t, i);
+    T fr[kEbFields], fi[kEbFields];
+    if (active) {
+      const uint64_t idx = (uint64_t)row * a.nx + x;
+      if (morph == 1) tensor_fields<T, S, 1>(vox, a.mats, a.M, a.V, idx, a.Q, rot, fr, fi);
+      else tensor_fields<T, S, 0>(vox, a.mats, a.M, a.V, idx, a.Q, rot, fr, fi);
+      const T w = hann ? a.hx[x] * wyz : wyz;
 #pragma unroll
       for (int c = 0; c < kEbFields; c++) {
-        fre[c][i] = fr[c];
-        fim[c][i] = fi[c];
+        fr[c] *= w;
+        fi[c] *= w;
       }
+    } else {
+#pragma unroll
+      for (int c = 0; c < kEbFields; c++) fr[c] = fi[c] = 0;
     }
 #pragma unroll
     for (int c = 0; c < kEbFields; c++) {
-      cx<T> v[E];
-#pragma unroll
-      for (int i = 0; i < E; i++) v[i] = mk<T>(fre[c][i], fim[c][i]);
-      fft.run(v, t, p, lds, a.twx);
-      if (active) {
-        cx<T> *o = a.out[c] + (uint64_t)row * a.nx;
-#pragma unroll
-        for (int i = 0; i < E; i++) o[fft.out_pos(t, i)] = v[i];
-      }
-      // (run() ends with a barrier after its last LDS reads, so the next field's
-      // or row's first LDS store cannot overtake them)
+      fre[c][i] = fr[c];
+      fim[c][i] = fi[c];
     }
+  }
+#pragma unroll
+  for (int c = 0; c < kEbFields; c++) {
+    cx<T> v[E];
+#pragma unroll
+    for (int i = 0; i < E; i++) v[i] = mk<T>(fre[c][i], fim[c][i]);
+    fft.run(v, t, p, lds, a.twx);
+    if (active) {
+      cx<T> *o = a.out[c] + (uint64_t)row * a.nx;
+#pragma unroll
+      for (int i = 0; i < E; i++) o[fft.out_pos(t, i)] = v[i];
+    }
+    // (run() ends with a barrier after its last LDS reads, so the next field's
+    // first LDS store cannot overtake them)
   }
 }
 
@@ -671,13 +670,12 @@ cyr_status launch_ebasis_polx(PolxArgs<T> a, const void *vox, int storage, int m
   a.pp = wg / a.tpp;
   if (a.pp < 1) a.pp = 1;
   a.pitch = ct ? ct_row_pitch(a.nx) : a.nx;
-  a.rpb = 1;  // row groups per block
   const size_t lds = (size_t)a.pp * a.pitch * sizeof(cx<T>);
   if (lds > device_max_lds()) return fail(CYR_E_UNSUPPORTED, "E-basis: row does not fit in LDS");
   cyr_status s;
   a.twx = twiddles<T>(a.nx, &s);
   if (!a.twx) return s;
-  const dim3 grid(ceil_div(a.rows, a.pp * a.rpb)), block(a.pp * a.tpp);
+  const dim3 grid(ceil_div(a.rows, a.pp)), block(a.pp * a.tpp);
   bool found = false;
   hipError_t e = hipSuccess;
   auto go = [&](auto tag) {

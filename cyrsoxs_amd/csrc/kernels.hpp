@@ -85,7 +85,7 @@ template <typename T> struct PolxArgs {
   Mat3<T> Q;                       // columns e1, e2, k^ (row-major)
   int32_t rot;                     // 0 when Q is the identity
   const cx<T> *twx;                // filled by the launcher
-  uint32_t tpp, pp, pitch, rpb;    // filled by the launcher
+  uint32_t tpp, pp, pitch;         // filled by the launcher
   cx<T> *out[kEbFields];           // X-transformed tensor fields
 };
 template <typename T>

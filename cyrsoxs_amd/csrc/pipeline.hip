@@ -12,6 +12,7 @@
 // DFT only at each pixel's Ewald bin(s); all angles then become 2D work.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -166,74 +167,71 @@ template <typename T> Mat3<T> m3(const M3 &m) {
   return r;
 }
 
-template <typename T>
-cyr_status run_impl(const cyr_sim_desc *d, const SimGeometry &g, const void *d_vox, const double *mats, T *d_out,
-                    WorkBuffers &wb, hipStream_t st) {
-  const uint64_t V = g.V, P = g.P;
-  const bool ebasis = d->pipeline == CYR_PIPE_EBASIS && ebasis_usable(d, g);
-  const bool alg1 = d->algorithm == 1;
-  const bool hann = d->windowing == 1 && !alg1;  // Algorithm 1 has no window (include/uniaxial.h:441-521)
-  const bool matframe = d->reference_frame == 0;
-  const bool full = d->scatter_approach == 1 && !ebasis;
+// Constant tables and per-simulation setup shared by both executors.
+template <typename T> struct Setup {
+  const MatConst<T> *mats;
+  const T *hx, *hy, *hz, *amat;
+  const double *rotinv;
+  const cx<T> *twy, *twz;
+  double vox_bytes, cxb;
+};
 
-  Tables tb;
+template <typename T>
+cyr_status setup_tables(const cyr_sim_desc *d, const SimGeometry &g, const double *mats, WorkBuffers &wb,
+                        hipStream_t st, Tables &tb, Setup<T> &su) {
   build_tables<T>(d, g, mats, tb);
   CYR_TRY(ensure_buffer(wb.tables, tb.total));
   CYR_HIP(hipMemcpyAsync(wb.tables.ptr, tb.host.data(), tb.total, hipMemcpyHostToDevice, st));
   unsigned char *tbase = reinterpret_cast<unsigned char *>(wb.tables.ptr);
-  const MatConst<T> *d_mats = reinterpret_cast<const MatConst<T> *>(tbase + tb.mats);
-  const T *d_han = reinterpret_cast<const T *>(tbase + tb.han);
-  const T *d_amat = reinterpret_cast<const T *>(tbase + tb.amat);
-  const double *d_rotinv = reinterpret_cast<const double *>(tbase + tb.rotinv);
+  su.mats = reinterpret_cast<const MatConst<T> *>(tbase + tb.mats);
+  const T *han = reinterpret_cast<const T *>(tbase + tb.han);
+  su.hx = han;
+  su.hy = han + g.nx;
+  su.hz = han + g.nx + g.ny;
+  su.amat = reinterpret_cast<const T *>(tbase + tb.amat);
+  su.rotinv = reinterpret_cast<const double *>(tbase + tb.rotinv);
+  cyr_status s;
+  su.twz = g.nz > 1 ? twiddles<T>(g.nz, &s) : nullptr;
+  if (g.nz > 1 && !su.twz) return s;
+  su.twy = twiddles<T>(g.ny, &s);
+  if (!su.twy) return s;
+  if (!twiddles<T>(g.nx, &s)) return s;  // created before any work is queued
+  su.cxb = (double)sizeof(cx<T>);
+  const double vox_el = d->storage == CYR_STORE_NATIVE ? (double)sizeof(T) : 2.0;
+  su.vox_bytes = 4.0 * vox_el * g.M * g.V;
+  return CYR_OK;
+}
 
-  const uint32_t ncomp = ebasis ? kEbFields : 3;
-  CYR_TRY(ensure_buffer(wb.pol, sizeof(cx<T>) * V * ncomp));
-  cx<T> *pol = reinterpret_cast<cx<T> *>(wb.pol.ptr);
-  cx<T> *vols[kEbFields];
-  void *vptr[kEbFields];
-  for (uint32_t c = 0; c < ncomp; c++) {
+// PLAIN: the reference's per-angle structure (src/cudaMain.cu:586-971).
+template <typename T>
+cyr_status run_plain(const cyr_sim_desc *d, const SimGeometry &g, const void *d_vox, T *d_out, WorkBuffers &wb,
+                     hipStream_t st, const Setup<T> &su) {
+  const uint64_t V = g.V, P = g.P;
+  const bool alg1 = d->algorithm == 1;
+  const bool hann = d->windowing == 1 && !alg1;  // Algorithm 1 has no window (include/uniaxial.h:441-521)
+  const bool matframe = d->reference_frame == 0;
+  const bool full = d->scatter_approach == 1;
+  const int ng = (g.nz > 1 && d->interpolation == 1) ? 2 : 1;
+  LaneBuffers &lb = wb.lane[0];
+  CYR_TRY(ensure_buffer(lb.pol, sizeof(cx<T>) * V * 3));
+  cx<T> *pol = reinterpret_cast<cx<T> *>(lb.pol.ptr);
+  cx<T> *vols[3];
+  void *vptr[3];
+  for (uint32_t c = 0; c < 3; c++) {
     vols[c] = pol + (uint64_t)c * V;
     vptr[c] = vols[c];
   }
-  if (alg1 && !ebasis) CYR_TRY(ensure_buffer(wb.nt, sizeof(cx<T>) * V * 6));
+  if (alg1) CYR_TRY(ensure_buffer(wb.nt, sizeof(cx<T>) * V * 6));
   if (full) CYR_TRY(ensure_buffer(wb.scatter, sizeof(T) * V));
-  CYR_TRY(ensure_buffer(wb.proj, sizeof(T) * P * (ebasis ? g.nA : 1)));
+  CYR_TRY(ensure_buffer(lb.proj, sizeof(T) * P));
   CYR_TRY(ensure_buffer(wb.acc, sizeof(double) * P));
   CYR_TRY(ensure_buffer(wb.count, sizeof(uint32_t) * P));
-  CYR_TRY(ensure_buffer(wb.avg, sizeof(double) * P));
-
-  T *proj = reinterpret_cast<T *>(wb.proj.ptr);
+  CYR_TRY(ensure_buffer(lb.avg, sizeof(double) * P));
+  T *proj = reinterpret_cast<T *>(lb.proj.ptr);
   double *acc = reinterpret_cast<double *>(wb.acc.ptr);
   uint32_t *count = reinterpret_cast<uint32_t *>(wb.count.ptr);
-  double *avg = reinterpret_cast<double *>(wb.avg.ptr);
-  cyr_status s;
-  const cx<T> *twz = g.nz > 1 ? twiddles<T>(g.nz, &s) : nullptr;
-  if (g.nz > 1 && !twz) return s;
-  const cx<T> *twy = twiddles<T>(g.ny, &s);
-  if (!twy) return s;
-  // E-basis: pixel -> z-bin table and the Y/z plan (sizes the partial-sum buffer)
-  const int ng = (g.nz > 1 && d->interpolation == 1) ? 2 : 1;
-  int32_t *kzt = nullptr;
-  cx<T> *part = nullptr;
-  double2 *dcz = nullptr;
-  YzArgs<T> yz{};
-  if (ebasis) {
-    yz.nx = g.nx;
-    yz.ny = g.ny;
-    yz.nz = g.nz;
-    CYR_TRY(ebasis_yz_plan<T>(yz, 0));
-    CYR_TRY(ensure_buffer(wb.bins, sizeof(int32_t) * P));
-    CYR_TRY(ensure_buffer(wb.part, sizeof(cx<T>) * P * ng * kEbFields * yz.nchunks));
-    CYR_TRY(ensure_buffer(wb.dc, sizeof(double2) * kEbFields * g.nz));
-    kzt = reinterpret_cast<int32_t *>(wb.bins.ptr);
-    part = reinterpret_cast<cx<T> *>(wb.part.ptr);
-    dcz = reinterpret_cast<double2 *>(wb.dc.ptr);
-  }
-  // algorithmic HBM bytes of the timed launches (cyr_kernel_timing)
-  const double cxb = (double)sizeof(cx<T>);
-  const double vox_el = d->storage == CYR_STORE_NATIVE ? (double)sizeof(T) : 2.0;
-  const double vox_bytes = 4.0 * vox_el * g.M * V;
-  const double part_bytes = cxb * P * ng * kEbFields * yz.nchunks;
+  double *avg = reinterpret_cast<double *>(lb.avg.ptr);
+  const double cxb = su.cxb;
 
   PolarArgs<T> pa{};
   pa.M = g.M;
@@ -241,116 +239,187 @@ cyr_status run_impl(const cyr_sim_desc *d, const SimGeometry &g, const void *d_v
   pa.nx = g.nx;
   pa.ny = g.ny;
   pa.nz = g.nz;
-  pa.hx = d_han;
-  pa.hy = d_han + g.nx;
-  pa.hz = d_han + g.nx + g.ny;
-  for (uint32_t c = 0; c < ncomp; c++) pa.out[c] = vols[c];
+  pa.hx = su.hx;
+  pa.hy = su.hy;
+  pa.hz = su.hz;
+  for (uint32_t c = 0; c < 3; c++) pa.out[c] = vols[c];
 
   for (uint32_t e = 0; e < g.nE; e++) {
-    pa.mats = d_mats + (size_t)e * g.M;
-    if (alg1 && !ebasis)
+    pa.mats = su.mats + (size_t)e * g.M;
+    if (alg1)
       CYR_TRY(launch_nt<T>(pa, d_vox, d->storage, d->morphology_type, reinterpret_cast<cx<T> *>(wb.nt.ptr), st));
     for (uint32_t k = 0; k < g.nK; k++) {
       const double kv[3] = {g.kvec[k].x, g.kvec[k].y, g.kvec[k].z};
       const EwaldGeom geo = make_ewald_geom(g.nx, g.ny, g.nz, d->phys_size, g.kmag[e], kv, d->interpolation);
       T *out = d_out + ((uint64_t)e * g.nK + k) * P;
-      if (ebasis) {
-        // bins -> tensor fields + X FFT -> Y FFT + z bins (+ DC probes) -> angles -> rotation
+      CYR_TRY(fill<double>(acc, 0.0, P, st));
+      CYR_TRY(fill<uint32_t>(count, 0u, P, st));
+      for (uint32_t a = 0; a < g.nA; a++) {
+        const M3 &R = g.R[(size_t)k * g.nA + a];
+        pa.e[0] = Vec3<T>{(T)R.m[0], (T)R.m[3], (T)R.m[6]};  // E = R x^
+        pa.R = m3<T>(R);
         kt_begin(st);
-        CYR_TRY(launch_ebasis_bins(geo, kzt, st));
-        kt_end("ebasis_bins", st, (double)P * 4);
-        PolxArgs<T> xa{};
-        xa.mats = pa.mats;
-        xa.M = g.M;
-        xa.V = V;
-        xa.nx = g.nx;
-        xa.ny = g.ny;
-        xa.nz = g.nz;
-        xa.rows = g.ny * g.nz;
-        xa.hx = pa.hx;
-        xa.hy = pa.hy;
-        xa.hz = pa.hz;
-        const M3 Q = ebasis_frame(g, k);
-        xa.Q = m3<T>(Q);
-        xa.rot = is_identity(Q) ? 0 : 1;
-        for (int c = 0; c < kEbFields; c++) xa.out[c] = vols[c];
+        if (alg1)
+          CYR_TRY(launch_polarization_nt<T>(pa, reinterpret_cast<const cx<T> *>(wb.nt.ptr), 1, matframe, st));
+        else
+          CYR_TRY(launch_polarization<T>(pa, d_vox, d->storage, d->morphology_type, 1, hann, matframe, st));
+        kt_end(alg1 ? "polarization_nt" : "polarization", st, (alg1 ? 6.0 * V * cxb : su.vox_bytes) + 3.0 * V * cxb);
         kt_begin(st);
-        CYR_TRY(launch_ebasis_polx<T>(xa, d_vox, d->storage, d->morphology_type, hann, st));
-        kt_end("ebasis_polx", st, vox_bytes + kEbFields * (double)V * cxb);
-        DcArgs<T> da{};
-        for (int c = 0; c < kEbFields; c++) da.vol[c] = vols[c];
-        da.twy = twy;
-        da.twz = twz;
-        da.nx = g.nx;
-        da.ny = g.ny;
-        da.nz = g.nz;
-        da.dcz = dcz;
+        CYR_TRY(fft3d<T>(vptr, 3, g.nx, g.ny, g.nz, st));
+        kt_end("fft3d_3vol", st, (g.nz > 1 ? 3.0 : 2.0) * 3 * 2 * V * cxb);
         kt_begin(st);
-        CYR_TRY(launch_ebasis_dc<T>(da, st));
-        kt_end("ebasis_dc", st, 3.0 * kEbFields * (double)g.ny * g.nz * cxb);
-        YzArgs<T> ya = yz;
-        for (int c = 0; c < kEbFields; c++) ya.vol[c] = vols[c];
-        ya.part = part;
-        ya.kzt = g.nz > 1 ? kzt : nullptr;
-        ya.twz = twz;
-        kt_begin(st);
-        CYR_TRY(launch_ebasis_yz<T>(ya, ng, st));
-        kt_end("ebasis_yz", st, kEbFields * (double)V * cxb + (g.nz > 1 ? (double)P * 4 : 0.0) + part_bytes);
-        AnglesArgs<T> aa{};
-        aa.g = geo;
-        aa.part = part;
-        aa.nchunks = yz.nchunks;
-        aa.dcz = dcz;
-        aa.amat = d_amat + (size_t)kEbAmat * k * g.nA;
-        aa.nA = g.nA;
-        aa.proj = proj;
-        kt_begin(st);
-        CYR_TRY(launch_ebasis_angles<T>(aa, ng, st));
-        kt_end("ebasis_angles", st, part_bytes + (double)g.nA * P * sizeof(T));
-        kt_begin(st);
-        CYR_TRY(launch_rotate_all<T>(proj, d_rotinv + (size_t)6 * k * g.nA, g.nA, g.nx, g.ny, d->rot_mask, avg, st));
-        kt_end("rotate_all", st, (double)(g.nA + 1) * P * sizeof(T));
-      } else {
-        CYR_TRY(fill<double>(acc, 0.0, P, st));
-        CYR_TRY(fill<uint32_t>(count, 0u, P, st));
-        for (uint32_t a = 0; a < g.nA; a++) {
-          const M3 &R = g.R[(size_t)k * g.nA + a];
-          pa.e[0] = Vec3<T>{(T)R.m[0], (T)R.m[3], (T)R.m[6]};  // E = R x^
-          pa.R = m3<T>(R);
+        CYR_TRY(launch_dc_replace<T>(vols, 3, g.nx, g.ny, g.nz, st));
+        kt_end("dc_replace", st);
+        if (full) {
+          T *sc = reinterpret_cast<T *>(wb.scatter.ptr);
+          CYR_TRY(launch_scatter3d<T>(geo, vols[0], vols[1], vols[2], sc, st));
+          CYR_TRY(launch_ewald_full<T>(geo, sc, proj, st));
+        } else {
           kt_begin(st);
-          if (alg1)
-            CYR_TRY(launch_polarization_nt<T>(pa, reinterpret_cast<const cx<T> *>(wb.nt.ptr), 1, matframe, st));
-          else
-            CYR_TRY(launch_polarization<T>(pa, d_vox, d->storage, d->morphology_type, 1, hann, matframe, st));
-          kt_end(alg1 ? "polarization_nt" : "polarization", st, (alg1 ? 6.0 * V * cxb : vox_bytes) + 3.0 * V * cxb);
-          kt_begin(st);
-          CYR_TRY(fft3d<T>(vptr, 3, g.nx, g.ny, g.nz, st));
-          kt_end("fft3d_3vol", st, (g.nz > 1 ? 3.0 : 2.0) * 3 * 2 * V * cxb);
-          kt_begin(st);
-          CYR_TRY(launch_dc_replace<T>(vols, 3, g.nx, g.ny, g.nz, st));
-          kt_end("dc_replace", st);
-          if (full) {
-            T *sc = reinterpret_cast<T *>(wb.scatter.ptr);
-            CYR_TRY(launch_scatter3d<T>(geo, vols[0], vols[1], vols[2], sc, st));
-            CYR_TRY(launch_ewald_full<T>(geo, sc, proj, st));
-          } else {
-            kt_begin(st);
-            CYR_TRY(launch_ewald_partial<T>(geo, vols[0], vols[1], vols[2], proj, st));
-            kt_end("ewald_partial", st, 3.0 * ng * P * cxb + (double)P * sizeof(T));
-          }
-          kt_begin(st);
-          CYR_TRY(launch_rotate_accumulate<T>(proj, acc, count, g.nx, g.ny, &g.rot_inv[((size_t)k * g.nA + a) * 6],
-                                              d->rot_mask, st));
-          kt_end("rotate_accumulate", st, (double)P * (sizeof(T) + 2 * (8 + 4)));
+          CYR_TRY(launch_ewald_partial<T>(geo, vols[0], vols[1], vols[2], proj, st));
+          kt_end("ewald_partial", st, 3.0 * ng * P * cxb + (double)P * sizeof(T));
         }
-        CYR_TRY(launch_average(acc, count, avg, P, g.nA, d->rot_mask, st));
+        kt_begin(st);
+        CYR_TRY(launch_rotate_accumulate<T>(proj, acc, count, g.nx, g.ny, &g.rot_inv[((size_t)k * g.nA + a) * 6],
+                                            d->rot_mask, st));
+        kt_end("rotate_accumulate", st, (double)P * (sizeof(T) + 2 * (8 + 4)));
       }
+      CYR_TRY(launch_average(acc, count, avg, P, g.nA, d->rot_mask, st));
       CYR_TRY(launch_detector_warp<T>(avg, out, g.nx, g.ny, &g.det_inv[(size_t)k * 6], d->rot_mask, st));
     }
   }
-  // keep the host table alive until the upload has been consumed
-  CYR_HIP(hipStreamSynchronize(st));
+  return CYR_OK;
+}
+
+// EBASIS: per (energy, k) unit the five tensor fields, their XY transforms, the
+// z DFT at the Ewald bins, then every angle as 2D work.  With two lanes, unit u
+// runs on stream u % 2 with its own work buffers and starts once unit u-1's
+// X kernel has finished, so one unit's X kernel (HBM-heavy) overlaps the
+// previous unit's Y/z chain (ALU/LDS-heavy).
+template <typename T>
+cyr_status run_ebasis(const cyr_sim_desc *d, const SimGeometry &g, const void *d_vox, T *d_out, WorkBuffers &wb,
+                      hipStream_t st, const Setup<T> &su) {
+  const uint64_t V = g.V, P = g.P;
+  const bool hann = d->windowing == 1 && d->algorithm != 1;
+  const int ng = (g.nz > 1 && d->interpolation == 1) ? 2 : 1;
+  const uint32_t units = g.nE * g.nK;
+  const uint32_t lanes = units > 1 ? energy_lanes() : 1;
+  YzArgs<T> yz{};
+  yz.nx = g.nx;
+  yz.ny = g.ny;
+  yz.nz = g.nz;
+  CYR_TRY(ebasis_yz_plan<T>(yz, 0));
+  for (uint32_t L = 0; L < lanes; L++) {
+    LaneBuffers &lb = wb.lane[L];
+    CYR_TRY(ensure_buffer(lb.pol, sizeof(cx<T>) * V * kEbFields));
+    CYR_TRY(ensure_buffer(lb.proj, sizeof(T) * P * g.nA));
+    CYR_TRY(ensure_buffer(lb.avg, sizeof(double) * P));
+    CYR_TRY(ensure_buffer(lb.bins, sizeof(int32_t) * P));
+    CYR_TRY(ensure_buffer(lb.part, sizeof(cx<T>) * P * ng * kEbFields * yz.nchunks));
+    CYR_TRY(ensure_buffer(lb.dc, sizeof(double2) * kEbFields * g.nz));
+  }
+  hipStream_t lane_st[2] = {st, st};
+  if (lanes > 1) {
+    CYR_TRY(wb.ensure_aux());
+    lane_st[1] = wb.aux;
+    CYR_HIP(hipEventRecord(wb.ev_fork, st));
+    CYR_HIP(hipStreamWaitEvent(wb.aux, wb.ev_fork, 0));
+  }
+  const double cxb = su.cxb;
+  const double part_bytes = cxb * P * ng * kEbFields * yz.nchunks;
+  for (uint32_t u = 0; u < units; u++) {
+    const uint32_t e = u / g.nK, k = u % g.nK, L = u % lanes;
+    hipStream_t s = lane_st[L];
+    LaneBuffers &lb = wb.lane[L];
+    cx<T> *pol = reinterpret_cast<cx<T> *>(lb.pol.ptr);
+    int32_t *kzt = reinterpret_cast<int32_t *>(lb.bins.ptr);
+    cx<T> *part = reinterpret_cast<cx<T> *>(lb.part.ptr);
+    double2 *dcz = reinterpret_cast<double2 *>(lb.dc.ptr);
+    T *proj = reinterpret_cast<T *>(lb.proj.ptr);
+    double *avg = reinterpret_cast<double *>(lb.avg.ptr);
+    if (lanes > 1 && u > 0) CYR_HIP(hipStreamWaitEvent(s, wb.ev_polx[(u - 1) % lanes], 0));
+    const double kv[3] = {g.kvec[k].x, g.kvec[k].y, g.kvec[k].z};
+    const EwaldGeom geo = make_ewald_geom(g.nx, g.ny, g.nz, d->phys_size, g.kmag[e], kv, d->interpolation);
+    T *out = d_out + ((uint64_t)e * g.nK + k) * P;
+    // bins -> tensor fields + X FFT -> DC neighbours -> Y FFT + z bins -> angles -> rotation
+    kt_begin(s);
+    CYR_TRY(launch_ebasis_bins(geo, kzt, s));
+    kt_end("ebasis_bins", s, (double)P * 4);
+    PolxArgs<T> xa{};
+    xa.mats = su.mats + (size_t)e * g.M;
+    xa.M = g.M;
+    xa.V = V;
+    xa.nx = g.nx;
+    xa.ny = g.ny;
+    xa.nz = g.nz;
+    xa.rows = g.ny * g.nz;
+    xa.hx = su.hx;
+    xa.hy = su.hy;
+    xa.hz = su.hz;
+    const M3 Q = ebasis_frame(g, k);
+    xa.Q = m3<T>(Q);
+    xa.rot = is_identity(Q) ? 0 : 1;
+    for (int c = 0; c < kEbFields; c++) xa.out[c] = pol + (uint64_t)c * V;
+    kt_begin(s);
+    CYR_TRY(launch_ebasis_polx<T>(xa, d_vox, d->storage, d->morphology_type, hann, s));
+    kt_end("ebasis_polx", s, su.vox_bytes + kEbFields * (double)V * cxb);
+    if (lanes > 1) CYR_HIP(hipEventRecord(wb.ev_polx[L], s));
+    DcArgs<T> da{};
+    for (int c = 0; c < kEbFields; c++) da.vol[c] = pol + (uint64_t)c * V;
+    da.twy = su.twy;
+    da.twz = su.twz;
+    da.nx = g.nx;
+    da.ny = g.ny;
+    da.nz = g.nz;
+    da.dcz = dcz;
+    kt_begin(s);
+    CYR_TRY(launch_ebasis_dc<T>(da, s));
+    kt_end("ebasis_dc", s, 3.0 * kEbFields * (double)g.ny * g.nz * cxb);
+    YzArgs<T> ya = yz;
+    for (int c = 0; c < kEbFields; c++) ya.vol[c] = pol + (uint64_t)c * V;
+    ya.part = part;
+    ya.kzt = g.nz > 1 ? kzt : nullptr;
+    ya.twz = su.twz;
+    kt_begin(s);
+    CYR_TRY(launch_ebasis_yz<T>(ya, ng, s));
+    kt_end("ebasis_yz", s, kEbFields * (double)V * cxb + (g.nz > 1 ? (double)P * 4 : 0.0) + part_bytes);
+    AnglesArgs<T> aa{};
+    aa.g = geo;
+    aa.part = part;
+    aa.nchunks = yz.nchunks;
+    aa.dcz = dcz;
+    aa.amat = su.amat + (size_t)kEbAmat * k * g.nA;
+    aa.nA = g.nA;
+    aa.proj = proj;
+    kt_begin(s);
+    CYR_TRY(launch_ebasis_angles<T>(aa, ng, s));
+    kt_end("ebasis_angles", s, part_bytes + (double)g.nA * P * sizeof(T));
+    kt_begin(s);
+    CYR_TRY(launch_rotate_all<T>(proj, su.rotinv + (size_t)6 * k * g.nA, g.nA, g.nx, g.ny, d->rot_mask, avg, s));
+    kt_end("rotate_all", s, (double)(g.nA + 1) * P * sizeof(T));
+    CYR_TRY(launch_detector_warp<T>(avg, out, g.nx, g.ny, &g.det_inv[(size_t)k * 6], d->rot_mask, s));
+  }
+  if (lanes > 1) {
+    CYR_HIP(hipEventRecord(wb.ev_join, wb.aux));
+    CYR_HIP(hipStreamWaitEvent(st, wb.ev_join, 0));
+  }
+  return CYR_OK;
+}
+
+template <typename T>
+cyr_status run_impl(const cyr_sim_desc *d, const SimGeometry &g, const void *d_vox, const double *mats, T *d_out,
+                    WorkBuffers &wb, hipStream_t st) {
+  Tables tb;
+  Setup<T> su{};
+  CYR_TRY(setup_tables<T>(d, g, mats, wb, st, tb, su));
+  const bool ebasis = d->pipeline == CYR_PIPE_EBASIS && ebasis_usable(d, g);
+  const cyr_status s = ebasis ? run_ebasis<T>(d, g, d_vox, d_out, wb, st, su) : run_plain<T>(d, g, d_vox, d_out, wb, st, su);
+  // keep the host table alive until the upload has been consumed (and drain the
+  // stream on errors too, so no queued launch outlives the call's buffers)
+  const hipError_t se = hipStreamSynchronize(st);
   kt_collect();
+  if (s != CYR_OK) return s;
+  if (se != hipSuccess) return hip_fail(se, "hipStreamSynchronize", __FILE__, __LINE__);
   return CYR_OK;
 }
 
@@ -386,24 +455,53 @@ cyr_status device_state(DeviceState **out) {
 
 }  // namespace
 
-void WorkBuffers::release() {
+void LaneBuffers::release() {
   free_buffer(pol);
-  free_buffer(nt);
-  free_buffer(scatter);
   free_buffer(proj);
-  free_buffer(acc);
-  free_buffer(count);
   free_buffer(avg);
-  free_buffer(tables);
-  free_buffer(dc);
   free_buffer(bins);
   free_buffer(part);
+  free_buffer(dc);
+}
+
+size_t LaneBuffers::bytes() const { return pol.bytes + proj.bytes + avg.bytes + bins.bytes + part.bytes + dc.bytes; }
+
+cyr_status WorkBuffers::ensure_aux() {
+  if (aux) return CYR_OK;
+  CYR_HIP(hipStreamCreateWithFlags(&aux, hipStreamNonBlocking));  // default priority
+  CYR_HIP(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+  CYR_HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+  for (auto &ev : ev_polx) CYR_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  return CYR_OK;
+}
+
+void WorkBuffers::release() {
+  for (auto &l : lane) l.release();
+  free_buffer(nt);
+  free_buffer(scatter);
+  free_buffer(acc);
+  free_buffer(count);
+  free_buffer(tables);
+  if (aux) {
+    (void)hipStreamSynchronize(aux);
+    (void)hipStreamDestroy(aux);
+    aux = nullptr;
+    (void)hipEventDestroy(ev_fork);
+    (void)hipEventDestroy(ev_join);
+    for (auto &ev : ev_polx) (void)hipEventDestroy(ev);
+  }
 }
 
 size_t WorkBuffers::bytes() const {
-  return pol.bytes + nt.bytes + scatter.bytes + proj.bytes + acc.bytes + count.bytes + avg.bytes + tables.bytes +
-         dc.bytes + bins.bytes + part.bytes;
+  return lane[0].bytes() + lane[1].bytes() + nt.bytes + scatter.bytes + acc.bytes + count.bytes + tables.bytes;
 }
+
+namespace {
+std::atomic<uint32_t> g_lanes{1};
+}  // namespace
+
+uint32_t energy_lanes() { return g_lanes.load(std::memory_order_relaxed); }
+void set_energy_lanes(uint32_t n) { g_lanes.store(n < 1 ? 1 : (n > 2 ? 2 : n)); }
 
 uint64_t morphology_bytes(const cyr_sim_desc *d) {
   const uint64_t V = (uint64_t)d->dims[0] * d->dims[1] * d->dims[2];
@@ -418,14 +516,18 @@ uint64_t output_bytes(const cyr_sim_desc *d, const SimGeometry &g) {
 uint64_t work_bytes(const cyr_sim_desc *d, const SimGeometry &g) {
   const uint64_t cs = d->precision == CYR_FP64 ? 16 : 8, rs = cs / 2;
   const bool ebasis = d->pipeline == CYR_PIPE_EBASIS && ebasis_usable(d, g);
-  uint64_t b = cs * g.V * (ebasis ? kEbFields : 3);
-  if (ebasis) {  // pixel -> bin table, Y/z partial sums (chunks x 2 bins x fields), DC probes
+  uint64_t b = 0;
+  if (ebasis) {  // per lane: fields, bin table, Y/z partial sums (chunks x 2 bins x fields), DC sums,
+                 // per-angle projections, average
     const uint64_t nch = ebasis_max_chunks(g.nx, g.ny, g.nz, d->precision);
-    b += 4 * g.P + cs * g.P * 2 * kEbFields * nch + 16 * kEbFields * g.nz;
+    const uint64_t lanes = (uint64_t)g.nE * g.nK > 1 ? energy_lanes() : 1;
+    b += lanes * (cs * g.V * kEbFields + 4 * g.P + cs * g.P * 2 * kEbFields * nch + 16 * kEbFields * g.nz +
+                  rs * g.P * g.nA + 8 * g.P);
+  } else {
+    b += cs * g.V * 3 + rs * g.P + 8 * g.P * 2 + 4 * g.P;
+    if (d->algorithm == 1) b += cs * g.V * 6;
+    if (d->scatter_approach == 1) b += rs * g.V;
   }
-  if (!ebasis && d->algorithm == 1) b += cs * g.V * 6;
-  if (!ebasis && d->scatter_approach == 1) b += rs * g.V;
-  b += rs * g.P * (ebasis ? g.nA : 1) + 8 * g.P * 2 + 4 * g.P;
   b += 64 * 1024 + (uint64_t)g.nK * g.nA * (16 + 72 + 48) + (uint64_t)g.nE * g.M * 64;
   return b;
 }
@@ -470,12 +572,13 @@ cyr_status autotune_impl(const cyr_sim_desc *d, const SimGeometry &g, WorkBuffer
   if (!ebasis || g.nz < 2) return CYR_OK;  // 2D: one slice, nothing to split
   const uint64_t V = g.V, P = g.P;
   const int ng = d->interpolation == 1 ? 2 : 1;
-  CYR_TRY(ensure_buffer(wb.pol, sizeof(cx<T>) * V * kEbFields));
-  CYR_HIP(hipMemsetAsync(wb.pol.ptr, 0, sizeof(cx<T>) * V * kEbFields, st));
-  CYR_TRY(ensure_buffer(wb.bins, sizeof(int32_t) * P));
+  LaneBuffers &lb = wb.lane[0];
+  CYR_TRY(ensure_buffer(lb.pol, sizeof(cx<T>) * V * kEbFields));
+  CYR_HIP(hipMemsetAsync(lb.pol.ptr, 0, sizeof(cx<T>) * V * kEbFields, st));
+  CYR_TRY(ensure_buffer(lb.bins, sizeof(int32_t) * P));
   const double kv[3] = {g.kvec[0].x, g.kvec[0].y, g.kvec[0].z};
   const EwaldGeom geo = make_ewald_geom(g.nx, g.ny, g.nz, d->phys_size, g.kmag[0], kv, d->interpolation);
-  CYR_TRY(launch_ebasis_bins(geo, reinterpret_cast<int32_t *>(wb.bins.ptr), st));
+  CYR_TRY(launch_ebasis_bins(geo, reinterpret_cast<int32_t *>(lb.bins.ptr), st));
   cyr_status s;
   const cx<T> *twz = twiddles<T>(g.nz, &s);
   if (!twz) return s;
@@ -495,11 +598,11 @@ cyr_status autotune_impl(const cyr_sim_desc *d, const SimGeometry &g, WorkBuffer
       ya.nz = g.nz;
       ya.pp = cols[ci];
       if (ebasis_yz_plan<T>(ya, targets[ti]) != CYR_OK || ya.pp != cols[ci]) continue;
-      rc = ensure_buffer(wb.part, sizeof(cx<T>) * P * ng * kEbFields * ya.nchunks);
+      rc = ensure_buffer(lb.part, sizeof(cx<T>) * P * ng * kEbFields * ya.nchunks);
       if (rc != CYR_OK) break;
-      for (int c = 0; c < kEbFields; c++) ya.vol[c] = reinterpret_cast<cx<T> *>(wb.pol.ptr) + (uint64_t)c * V;
-      ya.part = reinterpret_cast<cx<T> *>(wb.part.ptr);
-      ya.kzt = reinterpret_cast<int32_t *>(wb.bins.ptr);
+      for (int c = 0; c < kEbFields; c++) ya.vol[c] = reinterpret_cast<cx<T> *>(lb.pol.ptr) + (uint64_t)c * V;
+      ya.part = reinterpret_cast<cx<T> *>(lb.part.ptr);
+      ya.kzt = reinterpret_cast<int32_t *>(lb.bins.ptr);
       ya.twz = twz;
       rc = launch_ebasis_yz<T>(ya, ng, st);  // warm-up
       if (rc != CYR_OK) break;
@@ -549,6 +652,14 @@ cyr_status cyr_get_tuning(const uint32_t dims[3], int32_t precision, cyr_yz_tuni
   t->target_blocks = y.target_blocks;
   return CYR_OK;
 }
+
+cyr_status cyr_set_energy_lanes(uint32_t lanes) {
+  if (lanes < 1 || lanes > 2) return fail(CYR_E_INVALID, "energy lanes must be 1 or 2");
+  set_energy_lanes(lanes);
+  return CYR_OK;
+}
+
+uint32_t cyr_energy_lanes(void) { return energy_lanes(); }
 
 cyr_status cyr_autotune(const cyr_sim_desc *desc, cyr_yz_tuning *chosen) {
   SimGeometry g;

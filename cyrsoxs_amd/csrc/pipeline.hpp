@@ -7,11 +7,29 @@
 
 namespace cyr {
 
-struct WorkBuffers {
-  DeviceBuffer pol, nt, scatter, proj, acc, count, avg, tables, dc, bins, part;
+// Work buffers of one in-flight (energy, k) unit: polarization volumes (plain)
+// or tensor fields (E-basis), projections, average, and the E-basis bin table,
+// partial sums and DC sums.
+struct LaneBuffers {
+  DeviceBuffer pol, proj, avg, bins, part, dc;
   void release();
   size_t bytes() const;
 };
+
+struct WorkBuffers {
+  LaneBuffers lane[2];
+  DeviceBuffer nt, scatter, acc, count, tables;
+  // second stream (default priority) and events of the two-lane E-basis executor
+  hipStream_t aux = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_polx[2] = {nullptr, nullptr};
+  cyr_status ensure_aux();
+  void release();
+  size_t bytes() const;
+};
+
+// (energy, k) units of the E-basis executor in flight (1 or 2, default 1).
+uint32_t energy_lanes();
+void set_energy_lanes(uint32_t n);
 
 // Device bytes a simulation needs beyond its morphology and output.
 uint64_t work_bytes(const cyr_sim_desc *d, const SimGeometry &g);

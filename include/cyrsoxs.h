@@ -197,6 +197,14 @@ cyr_status cyr_get_tuning(const uint32_t dims[3], int32_t precision, cyr_yz_tuni
  * receives it ({0, 0} when the grid has nothing to tune, e.g. 2D). */
 cyr_status cyr_autotune(const cyr_sim_desc *desc, cyr_yz_tuning *chosen);
 
+/* (energy, k) units the E-basis executor keeps in flight on separate streams,
+ * each with its own work buffers: 1 (default, serial) or 2 (one unit's
+ * polarization/X-FFT kernel may overlap the previous unit's Y/z chain; pays off
+ * only when single kernels do not fill the device).  Results are identical
+ * either way.  No counterpart in the reference. */
+cyr_status cyr_set_energy_lanes(uint32_t lanes);
+uint32_t cyr_energy_lanes(void);
+
 /* ---- utilities ------------------------------------------------------------- */
 
 const char *cyr_status_string(cyr_status s);

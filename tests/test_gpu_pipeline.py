@@ -139,3 +139,47 @@ def test_half_precision_morphology_storage(storage, tol):
         err, same = pattern_rel_l2(got, ref)
         print(f"storage={storage} pipeline={pipeline}: rel L2 {err:.3e}")
         assert same and err < tol
+
+
+def test_energy_lanes_bitwise():
+    """One or two (energy, k) units in flight give bit-identical patterns."""
+    c = synthetic.config0()
+    sim = capi.Sim(c["dims"], 2, c["phys_size"], c["energies"], c["angles"], interpolation=1,
+                   pipeline=capi.PIPE_EBASIS)
+    lanes = capi.energy_lanes()
+    try:
+        capi.set_energy_lanes(1)
+        a = capi.simulate(sim, c["voxels"], c["mats"])
+        capi.set_energy_lanes(2)
+        b = capi.simulate(sim, c["voxels"], c["mats"])
+    finally:
+        capi.set_energy_lanes(lanes)
+    np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("columns,target", [(4, 256), (8, 2048), (16, 512)])
+def test_yz_tuning_changes_rounding_only(columns, target):
+    """Every Y/z launch shape the autotuner may pick meets the FP32 bar."""
+    c = synthetic.config0()
+    kw = dict(interpolation=1, windowing=0)
+    osim = O.Sim(c["dims"], 2, c["phys_size"], c["energies"], c["angles"], **kw)
+    ref = _oracle("sphere_linear", osim, c["voxels"], c["mats"])
+    sim = capi.Sim(c["dims"], 2, c["phys_size"], c["energies"], c["angles"], pipeline=capi.PIPE_EBASIS, **kw)
+    try:
+        capi.set_tuning(c["dims"], capi.FP32, columns, target)
+        got = capi.simulate(sim, c["voxels"], c["mats"])
+    finally:
+        capi.set_tuning(c["dims"], capi.FP32, 0, 0)
+    _check(got, ref, capi.FP32)
+
+
+def test_autotune_records_a_choice():
+    c = synthetic.config0()
+    sim = capi.Sim(c["dims"], 2, c["phys_size"], c["energies"], c["angles"], interpolation=1,
+                   pipeline=capi.PIPE_EBASIS)
+    try:
+        cols, target = capi.autotune(sim)
+        assert cols in (4, 8, 16) and target > 0
+        assert capi.get_tuning(c["dims"], capi.FP32) == (cols, target)
+    finally:
+        capi.set_tuning(c["dims"], capi.FP32, 0, 0)
