@@ -96,6 +96,13 @@ _SIGS = {
     "cyr_engine_cancel": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64]),
     "cyr_engine_get_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "cyr_engine_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "cyr_engine_request_times": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p,
+                                                ctypes.c_void_p]),
+    "cyr_engine_release": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64]),
+    "cyr_batch_create": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                        ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p]),
+    "cyr_batch_launch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "cyr_batch_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "cyr_status_string": (ctypes.c_char_p, [ctypes.c_int]),
     "cyr_last_error": (ctypes.c_char_p, []),
     "cyr_device_count": (ctypes.c_int, []),
@@ -313,6 +320,111 @@ def simulate_batch(sims, d_voxels, mats, d_outs, stream=None):
     mp = (ctypes.c_void_p * n)(*[m.ctypes.data for m in mats_c])
     op = (ctypes.c_void_p * n)(*[o.data_ptr() for o in d_outs])
     check(lib().cyr_simulate_batch(n, descs, vp, mp, op, _stream_ptr(stream)), "cyr_simulate_batch")
+
+
+class BatchPlan:
+    """A batch captured once in a hipGraph (cyr_batch_create) and replayed with
+    launch(); the torch tensors given here are bound for its lifetime."""
+
+    def __init__(self, sims, d_voxels, mats, d_outs, lanes=0):
+        n = len(sims)
+        self._keep = (sims, d_voxels, d_outs)
+        descs = (SimDesc * n)(*[s.desc for s in sims])
+        mats_c = [np.ascontiguousarray(m, dtype=np.float64) for m in mats]
+        vp = (ctypes.c_void_p * n)(*[v.data_ptr() for v in d_voxels])
+        mp = (ctypes.c_void_p * n)(*[m.ctypes.data for m in mats_c])
+        op = (ctypes.c_void_p * n)(*[o.data_ptr() for o in d_outs])
+        self.handle = ctypes.c_void_p()
+        check(lib().cyr_batch_create(n, descs, vp, mp, op, int(lanes), ctypes.byref(self.handle)), "cyr_batch_create")
+
+    def launch(self, stream=None):
+        check(lib().cyr_batch_launch(self.handle, _stream_ptr(stream)), "cyr_batch_launch")
+
+    def close(self):
+        if self.handle:
+            check(lib().cyr_batch_destroy(self.handle), "cyr_batch_destroy")
+            self.handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class EngineStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64), ("rejected", ctypes.c_uint64),
+                ("cancelled", ctypes.c_uint64), ("resident_bytes", ctypes.c_uint64), ("budget_bytes", ctypes.c_uint64),
+                ("in_flight", ctypes.c_uint32), ("queued", ctypes.c_uint32)]
+
+
+class Engine:
+    """Serving engine (cyr_engine_*): submit() returns a request id; host inputs
+    and outputs are numpy arrays kept alive here until the request is released."""
+
+    def __init__(self, device=0, budget_bytes=0, slots=4):
+        self.handle = ctypes.c_void_p()
+        check(lib().cyr_engine_create(int(device), int(budget_bytes), int(slots), ctypes.byref(self.handle)),
+              "cyr_engine_create")
+        self._live = {}
+
+    def submit(self, sim: Sim, voxels, mats, out=None):
+        rd = sim.real_dtype
+        vox = np.ascontiguousarray(voxels, dtype=rd)
+        m = np.ascontiguousarray(mats, dtype=np.float64)
+        nx, ny, nz = sim.dims
+        if out is None:
+            out = np.empty((len(sim.energies), sim.num_k, ny, nx), dtype=rd)
+        rid = ctypes.c_uint64()
+        check(lib().cyr_engine_submit(self.handle, sim.ref, _ptr(vox), _ptr(m), _ptr(out), ctypes.byref(rid)),
+              "cyr_engine_submit")
+        self._live[rid.value] = (sim, vox, m, out)
+        return rid.value
+
+    def wait(self, rid, timeout=-1.0):
+        st = ctypes.c_int()
+        check(lib().cyr_engine_wait(self.handle, ctypes.c_uint64(rid), float(timeout), ctypes.byref(st)),
+              "cyr_engine_wait")
+        return int(st.value)
+
+    def poll(self, rid):
+        done, st = ctypes.c_int(), ctypes.c_int()
+        check(lib().cyr_engine_poll(self.handle, ctypes.c_uint64(rid), ctypes.byref(done), ctypes.byref(st)),
+              "cyr_engine_poll")
+        return bool(done.value), int(st.value)
+
+    def result(self, rid):
+        return self._live[rid][3]
+
+    def times(self, rid):
+        q, r, l = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        check(lib().cyr_engine_request_times(self.handle, ctypes.c_uint64(rid), ctypes.byref(q), ctypes.byref(r),
+                                             ctypes.byref(l)), "cyr_engine_request_times")
+        return q.value, r.value, l.value
+
+    def release(self, rid):
+        check(lib().cyr_engine_release(self.handle, ctypes.c_uint64(rid)), "cyr_engine_release")
+        return self._live.pop(rid)[3]
+
+    def cancel(self, rid):
+        check(lib().cyr_engine_cancel(self.handle, ctypes.c_uint64(rid)), "cyr_engine_cancel")
+
+    def stats(self):
+        s = EngineStats()
+        check(lib().cyr_engine_get_stats(self.handle, ctypes.byref(s)), "cyr_engine_get_stats")
+        return {k: getattr(s, k) for k, _ in s._fields_}
+
+    def close(self):
+        if self.handle:
+            check(lib().cyr_engine_destroy(self.handle), "cyr_engine_destroy")
+            self.handle = ctypes.c_void_p()
+            self._live.clear()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 def kernel_timing(enable):

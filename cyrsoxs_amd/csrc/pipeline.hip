@@ -12,6 +12,7 @@
 // DFT only at each pixel's Ewald bin(s); all angles then become 2D work.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <atomic>
 #include <cmath>
 #include <cstring>
@@ -25,6 +26,14 @@
 #include "pipeline.hpp"
 
 namespace cyr {
+
+// The fused E-basis kernels need 2^a 3^b 5^c in-plane lengths and tiles that fit
+// in LDS; FP64 runs them on native-precision morphologies only.  Other
+// simulations asking for E-basis run the plain executor.
+bool ebasis_usable(const cyr_sim_desc *d, const SimGeometry &g) {
+  return ebasis_supported(g.nx, g.ny, g.nz, d->precision) &&
+         !(d->precision == CYR_FP64 && d->storage != CYR_STORE_NATIVE);
+}
 
 namespace {
 
@@ -65,14 +74,6 @@ template <typename T> MatConst<T> mat_const(const double *m4) {
   r.dr = (T)(ar - br);
   r.di = (T)(ai - bi);
   return r;
-}
-
-// The fused E-basis kernels need 2^a 3^b 5^c in-plane lengths and tiles that fit
-// in LDS; FP64 runs them on native-precision morphologies only.  Other
-// simulations asking for E-basis run the plain executor.
-bool ebasis_usable(const cyr_sim_desc *d, const SimGeometry &g) {
-  return ebasis_supported(g.nx, g.ny, g.nz, d->precision) &&
-         !(d->precision == CYR_FP64 && d->storage != CYR_STORE_NATIVE);
 }
 
 // Per-simulation constant tables, uploaded once.
@@ -125,8 +126,8 @@ void ebasis_amat(const SimGeometry &g, uint32_t k, uint32_t a, bool matframe, do
     }
 }
 
-template <typename T>
-void build_tables(const cyr_sim_desc *d, const SimGeometry &g, const double *mats, Tables &tb) {
+// Offsets of the table sections (shared by upload and resident runs).
+template <typename T> void build_layout(const SimGeometry &g, Tables &tb) {
   size_t off = 0;
   tb.mats = off;
   off = align256(off + sizeof(MatConst<T>) * g.nE * g.M);
@@ -139,7 +140,12 @@ void build_tables(const cyr_sim_desc *d, const SimGeometry &g, const double *mat
   tb.detinv = off;
   off = align256(off + sizeof(double) * 6 * g.nK);
   tb.total = off;
-  tb.host.assign(off, 0);
+}
+
+template <typename T>
+void build_tables(const cyr_sim_desc *d, const SimGeometry &g, const double *mats, Tables &tb) {
+  build_layout<T>(g, tb);
+  tb.host.assign(tb.total, 0);
   MatConst<T> *mc = reinterpret_cast<MatConst<T> *>(tb.host.data() + tb.mats);
   for (uint32_t e = 0; e < g.nE; e++)
     for (uint32_t m = 0; m < g.M; m++) mc[(size_t)e * g.M + m] = mat_const<T>(mats + 4 * ((size_t)e * g.M + m));
@@ -176,13 +182,13 @@ template <typename T> struct Setup {
   double vox_bytes, cxb;
 };
 
+// Device views of the tables in wb.tables (laid out by build_tables) and the
+// twiddle tables; everything is created before any kernel is queued.
 template <typename T>
-cyr_status setup_tables(const cyr_sim_desc *d, const SimGeometry &g, const double *mats, WorkBuffers &wb,
-                        hipStream_t st, Tables &tb, Setup<T> &su) {
-  build_tables<T>(d, g, mats, tb);
-  CYR_TRY(ensure_buffer(wb.tables, tb.total));
-  CYR_HIP(hipMemcpyAsync(wb.tables.ptr, tb.host.data(), tb.total, hipMemcpyHostToDevice, st));
-  unsigned char *tbase = reinterpret_cast<unsigned char *>(wb.tables.ptr);
+cyr_status setup_views(const cyr_sim_desc *d, const SimGeometry &g, const Tables &tb, const void *tables,
+                       Setup<T> &su) {
+  if (!tables) return fail(CYR_E_INVALID, "tables not resident");
+  const unsigned char *tbase = reinterpret_cast<const unsigned char *>(tables);
   su.mats = reinterpret_cast<const MatConst<T> *>(tbase + tb.mats);
   const T *han = reinterpret_cast<const T *>(tbase + tb.han);
   su.hx = han;
@@ -195,7 +201,7 @@ cyr_status setup_tables(const cyr_sim_desc *d, const SimGeometry &g, const doubl
   if (g.nz > 1 && !su.twz) return s;
   su.twy = twiddles<T>(g.ny, &s);
   if (!su.twy) return s;
-  if (!twiddles<T>(g.nx, &s)) return s;  // created before any work is queued
+  if (!twiddles<T>(g.nx, &s)) return s;
   su.cxb = (double)sizeof(cx<T>);
   const double vox_el = d->storage == CYR_STORE_NATIVE ? (double)sizeof(T) : 2.0;
   su.vox_bytes = 4.0 * vox_el * g.M * g.V;
@@ -303,7 +309,7 @@ cyr_status run_ebasis(const cyr_sim_desc *d, const SimGeometry &g, const void *d
   const bool hann = d->windowing == 1 && d->algorithm != 1;
   const int ng = (g.nz > 1 && d->interpolation == 1) ? 2 : 1;
   const uint32_t units = g.nE * g.nK;
-  const uint32_t lanes = units > 1 ? energy_lanes() : 1;
+  const uint32_t lanes = units > 1 ? std::min(energy_lanes(), wb.max_lanes) : 1;
   YzArgs<T> yz{};
   yz.nx = g.nx;
   yz.ny = g.ny;
@@ -408,14 +414,23 @@ cyr_status run_ebasis(const cyr_sim_desc *d, const SimGeometry &g, const void *d
 
 template <typename T>
 cyr_status run_impl(const cyr_sim_desc *d, const SimGeometry &g, const void *d_vox, const double *mats, T *d_out,
-                    WorkBuffers &wb, hipStream_t st) {
+                    WorkBuffers &wb, hipStream_t st, RunMode mode, const void *resident_tables) {
   Tables tb;
+  const void *tables = resident_tables;
+  if (mode == kRunResident) {
+    build_layout<T>(g, tb);
+  } else {
+    build_tables<T>(d, g, mats, tb);
+    CYR_TRY(ensure_buffer(wb.tables, tb.total));
+    CYR_TRY(wb.upload(wb.tables.ptr, tb.host.data(), tb.total, st));
+    tables = wb.tables.ptr;
+  }
   Setup<T> su{};
-  CYR_TRY(setup_tables<T>(d, g, mats, wb, st, tb, su));
+  CYR_TRY(setup_views<T>(d, g, tb, tables, su));
   const bool ebasis = d->pipeline == CYR_PIPE_EBASIS && ebasis_usable(d, g);
   const cyr_status s = ebasis ? run_ebasis<T>(d, g, d_vox, d_out, wb, st, su) : run_plain<T>(d, g, d_vox, d_out, wb, st, su);
-  // keep the host table alive until the upload has been consumed (and drain the
-  // stream on errors too, so no queued launch outlives the call's buffers)
+  if (mode != kRunSync) return s;
+  // drain the stream on errors too, so no queued launch outlives the call's buffers
   const hipError_t se = hipStreamSynchronize(st);
   kt_collect();
   if (s != CYR_OK) return s;
@@ -475,7 +490,34 @@ cyr_status WorkBuffers::ensure_aux() {
   return CYR_OK;
 }
 
+cyr_status WorkBuffers::upload(void *dst, const void *src, size_t n, hipStream_t st) {
+  if (!stage_done) CYR_HIP(hipEventCreateWithFlags(&stage_done, hipEventDisableTiming));
+  if (stage_pending) {  // the previous upload must have left the staging buffer
+    CYR_HIP(hipEventSynchronize(stage_done));
+    stage_pending = false;
+  }
+  if (stage_bytes < n) {
+    if (stage) (void)hipHostFree(stage);
+    stage = nullptr;
+    stage_bytes = 0;
+    CYR_HIP(hipHostMalloc(&stage, n, hipHostMallocDefault));
+    stage_bytes = n;
+  }
+  std::memcpy(stage, src, n);
+  CYR_HIP(hipMemcpyAsync(dst, stage, n, hipMemcpyHostToDevice, st));
+  CYR_HIP(hipEventRecord(stage_done, st));
+  stage_pending = true;
+  return CYR_OK;
+}
+
 void WorkBuffers::release() {
+  if (stage_pending) (void)hipEventSynchronize(stage_done);
+  stage_pending = false;
+  if (stage) (void)hipHostFree(stage);
+  stage = nullptr;
+  stage_bytes = 0;
+  if (stage_done) (void)hipEventDestroy(stage_done);
+  stage_done = nullptr;
   for (auto &l : lane) l.release();
   free_buffer(nt);
   free_buffer(scatter);
@@ -533,11 +575,26 @@ uint64_t work_bytes(const cyr_sim_desc *d, const SimGeometry &g) {
 }
 
 cyr_status run_simulation(const cyr_sim_desc *d, const SimGeometry &g, const void *d_vox, const double *mats,
-                          void *d_out, WorkBuffers &wb, hipStream_t st) {
+                          void *d_out, WorkBuffers &wb, hipStream_t st, RunMode mode, const void *resident_tables) {
   if (d->precision == CYR_FP64)
-    return run_impl<double>(d, g, d_vox, mats, reinterpret_cast<double *>(d_out), wb, st);
-  return run_impl<float>(d, g, d_vox, mats, reinterpret_cast<float *>(d_out), wb, st);
+    return run_impl<double>(d, g, d_vox, mats, reinterpret_cast<double *>(d_out), wb, st, mode, resident_tables);
+  return run_impl<float>(d, g, d_vox, mats, reinterpret_cast<float *>(d_out), wb, st, mode, resident_tables);
 }
+
+size_t table_bytes(const cyr_sim_desc *d, const SimGeometry &g) {
+  Tables tb;
+  if (d->precision == CYR_FP64) build_layout<double>(g, tb);
+  else build_layout<float>(g, tb);
+  return tb.total;
+}
+
+void fill_tables(const cyr_sim_desc *d, const SimGeometry &g, const double *mats, void *host) {
+  Tables tb;
+  if (d->precision == CYR_FP64) build_tables<double>(d, g, mats, tb);
+  else build_tables<float>(d, g, mats, tb);
+  std::memcpy(host, tb.host.data(), tb.total);
+}
+
 
 void convert_storage(const cyr_sim_desc *d, const void *host_vox, std::vector<unsigned char> &out) {
   const uint64_t n = (uint64_t)d->dims[0] * d->dims[1] * d->dims[2] * d->num_materials * 4;
@@ -627,6 +684,12 @@ cyr_status autotune_impl(const cyr_sim_desc *d, const SimGeometry &g, WorkBuffer
   if (best >= 0) ebasis_set_yz_tuning(g.nx, g.ny, g.nz, d->precision, pick);
   *chosen = pick;
   return CYR_OK;
+}
+
+cyr_status autotune_shape(const cyr_sim_desc *d, const SimGeometry &g, WorkBuffers &wb, hipStream_t st,
+                          YzTuning *chosen) {
+  if (d->precision == CYR_FP64) return autotune_impl<double>(d, g, wb, st, chosen);
+  return autotune_impl<float>(d, g, wb, st, chosen);
 }
 
 }  // namespace cyr
@@ -742,7 +805,7 @@ cyr_status cyr_polarization_device(const cyr_sim_desc *desc, const void *d_voxel
     for (uint32_t e = 0; e < g.nE; e++) std::memcpy(&m[4 * (size_t)e * g.M], mats_e, sizeof(double) * 4 * g.M);
     build_tables<T>(desc, g, m.data(), tb);
     CYR_TRY(ensure_buffer(ds->wb.tables, tb.total));
-    CYR_HIP(hipMemcpyAsync(ds->wb.tables.ptr, tb.host.data(), tb.total, hipMemcpyHostToDevice, st));
+    CYR_TRY(ds->wb.upload(ds->wb.tables.ptr, tb.host.data(), tb.total, st));
     unsigned char *tbase = reinterpret_cast<unsigned char *>(ds->wb.tables.ptr);
     PolarArgs<T> pa{};
     pa.mats = reinterpret_cast<const MatConst<T> *>(tbase + tb.mats);

@@ -141,13 +141,28 @@ cyr_status cyr_warp_affine_device(uint32_t nx, uint32_t ny, int32_t precision, c
 
 /* ---- batched execution ---------------------------------------------------- */
 
-/* Runs `count` independent simulations (uneven grids allowed) resident in HBM
- * as one schedule: per-member work is interleaved across streams and the
- * member's kernel sequence is captured once in a hipGraph per shape.  No
- * counterpart in the reference (it runs one simulation per call). */
+/* Runs `count` independent simulations (uneven grids allowed) resident in HBM as
+ * one schedule: members are spread over several streams (longest first onto the
+ * least-loaded stream, each stream with its own work buffers), their constant
+ * tables are uploaded together, and every E-basis grid shape without a tuning is
+ * autotuned first.  Returns after `stream` has finished.  No counterpart in the
+ * reference (it runs one simulation per call). */
 cyr_status cyr_simulate_batch(uint32_t count, const cyr_sim_desc *descs,
                               const void *const *d_voxels, const double *const *mats,
                               void *const *d_outs, void *stream);
+
+/* A batch plan: the same schedule, set up once (tables resident, shapes tuned,
+ * buffers sized by one eager run whose outputs land in d_outs) and captured in a
+ * hipGraph over `lanes` streams (0 = automatic).  The device pointers are bound
+ * at creation; cyr_batch_launch replays the graph on `stream` (for instance
+ * after the caller has refreshed the morphologies in place) and returns
+ * without waiting.  Replays of one plan must not overlap. */
+typedef struct cyr_batch_s *cyr_batch;
+cyr_status cyr_batch_create(uint32_t count, const cyr_sim_desc *descs, const void *const *d_voxels,
+                            const double *const *mats, void *const *d_outs, uint32_t lanes,
+                            cyr_batch *out);
+cyr_status cyr_batch_launch(cyr_batch b, void *stream);
+cyr_status cyr_batch_destroy(cyr_batch b);
 
 /* ---- serving engine ------------------------------------------------------- */
 
@@ -155,27 +170,42 @@ typedef struct cyr_engine_s *cyr_engine;
 
 typedef struct {
   uint64_t submitted, completed, rejected, cancelled;
-  uint64_t resident_bytes;   /* HBM currently reserved by admitted requests */
+  uint64_t resident_bytes;   /* HBM reserved by the requests now running */
   uint64_t budget_bytes;
   uint32_t in_flight;
   uint32_t queued;
 } cyr_engine_stats;
 
-/* Continuous-batching engine on one device: requests are admitted by HBM
- * footprint (cyr_footprint_bytes) against budget_bytes, each admitted request's
- * morphology stays resident across its energies, and up to `slots` requests
- * run concurrently on their own streams.  No counterpart in the reference. */
+/* Continuous-batching engine on one device: requests are admitted in arrival
+ * order once their HBM footprint (cyr_footprint_bytes) fits in budget_bytes
+ * (0 = 90% of the device memory) next to the requests already running; each of
+ * `slots` (1..16) host threads drives its own stream, so requests that arrive
+ * while others run start as soon as a slot and memory free up and their work
+ * overlaps on the device.  Each request's morphology stays resident in HBM
+ * across its energies.  No counterpart in the reference. */
 cyr_status cyr_engine_create(int device, uint64_t budget_bytes, uint32_t slots, cyr_engine *out);
-/* Copies the host inputs into the engine; `out` (host, [nE][nK][Ny][Nx]) must
- * stay valid until the request completes. */
+/* Queues a simulation.  desc and mats are copied; voxels (host, [M][V][4] of the
+ * compute precision) and out (host, [nE][nK][Ny][Nx]) are used in place and
+ * must stay valid until the request completes (pinned host memory makes the
+ * transfers asynchronous).  CYR_E_ADMISSION if the footprint exceeds the budget. */
 cyr_status cyr_engine_submit(cyr_engine eng, const cyr_sim_desc *desc, const void *voxels,
                              const double *mats, void *out, uint64_t *request_id);
+/* Waits up to timeout_s (< 0: forever; CYR_E_TIMEOUT if still in flight) and
+ * reports the request's own status (CYR_E_CANCELLED if it was cancelled). */
 cyr_status cyr_engine_wait(cyr_engine eng, uint64_t request_id, double timeout_s,
                            cyr_status *request_status);
 cyr_status cyr_engine_poll(cyr_engine eng, uint64_t request_id, int *done,
                            cyr_status *request_status);
+/* Time in the queue, time running (upload, simulation, download) and the whole
+ * submit-to-result latency of a completed request, in seconds. */
+cyr_status cyr_engine_request_times(cyr_engine eng, uint64_t request_id, double *queued_s,
+                                    double *run_s, double *latency_s);
+/* Forgets a finished request (its id becomes unknown). */
+cyr_status cyr_engine_release(cyr_engine eng, uint64_t request_id);
+/* Cancels a request that has not started (CYR_E_INVALID once it runs). */
 cyr_status cyr_engine_cancel(cyr_engine eng, uint64_t request_id);
 cyr_status cyr_engine_get_stats(cyr_engine eng, cyr_engine_stats *stats);
+/* Cancels queued requests, lets running ones finish, frees everything. */
 cyr_status cyr_engine_destroy(cyr_engine eng);
 
 /* ---- autotuning ------------------------------------------------------------ */
