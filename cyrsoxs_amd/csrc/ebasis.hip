@@ -738,8 +738,10 @@ template <typename T> cyr_status ebasis_yz_plan(YzArgs<T> &a, uint32_t target_bl
 template <typename T> cyr_status launch_ebasis_yz(YzArgs<T> a, int ng, hipStream_t st) {
   StagePlan P;
   uint32_t lcm;
-  if (!make_stage_plan(a.ny, P, lcm) || !a.e || !a.pp || !a.tpp || !a.zchunk || a.ny % a.e != 0 || a.e % lcm != 0 ||
-      a.tpp != a.ny / a.e)
+  // (the run-time plan's elements per thread are a multiple of its radix lcm; a
+  // compile-time plan has its own radix sequence)
+  if (!make_stage_plan(a.ny, P, lcm) || !a.e || !a.pp || !a.tpp || !a.zchunk || a.ny % a.e != 0 ||
+      (!a.ct && a.e % lcm != 0) || a.tpp != a.ny / a.e)
     return fail(CYR_E_INVALID, "E-basis: Y plan missing (call ebasis_yz_plan)");
   const int prec = sizeof(T) == 8 ? CYR_FP64 : CYR_FP32;
   if (a.ct && (!ct_usable(a.ny, prec) || a.e != ct_elements(a.ny) || !ct_yz_pp_ok(a.ny, a.pp)))

@@ -154,7 +154,39 @@ cyr_status queue_schedule(const cyr_sim_desc *descs, const void *const *d_voxels
 
 uint32_t default_lanes(uint32_t count) { return count < 4 ? (count ? count : 1) : 4; }
 
+// Streams and work buffers of cyr_simulate_batch, kept per device across calls
+// (grown on demand) so a call does not re-allocate them.
+struct BatchState {
+  std::mutex mu;
+  Lanes lanes;
+};
+std::mutex g_batch_mu;
+std::map<int, BatchState *> g_batch;
+
+cyr_status batch_state(uint32_t nl, BatchState **out) {
+  int dev = 0;
+  CYR_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_batch_mu);
+  BatchState *&b = g_batch[dev];
+  if (!b) b = new BatchState();
+  *out = b;
+  (void)nl;
+  return CYR_OK;
+}
+
 }  // namespace
+
+void release_batch_buffers() {
+  std::lock_guard<std::mutex> lk(g_batch_mu);
+  int cur = 0;
+  (void)hipGetDevice(&cur);
+  for (auto &kv : g_batch) {
+    std::lock_guard<std::mutex> l2(kv.second->mu);
+    (void)hipSetDevice(kv.first);
+    kv.second->lanes.destroy();
+  }
+  (void)hipSetDevice(cur);
+}
 
 // ---- batch plan --------------------------------------------------------------
 
@@ -189,9 +221,16 @@ cyr_status cyr_simulate_batch(uint32_t count, const cyr_sim_desc *descs, const v
   for (uint32_t i = 0; i < count; i++)
     if (!d_voxels[i] || !mats[i] || !d_outs[i]) return fail(CYR_E_INVALID, "null batch member buffer");
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  BatchState *bs;
+  CYR_TRY(batch_state(default_lanes(count), &bs));
+  std::lock_guard<std::mutex> lk(bs->mu);
+  Lanes &L = bs->lanes;
+  cyr_status rc = CYR_OK;
+  if (L.st.size() < default_lanes(count)) {
+    L.destroy();
+    rc = L.create(default_lanes(count));
+  }
   Schedule s;
-  Lanes L;
-  cyr_status rc = L.create(default_lanes(count));
   if (rc == CYR_OK) rc = plan_schedule(count, descs, mats, (uint32_t)L.st.size(), st, s);
   if (rc == CYR_OK) rc = tune_shapes(count, descs, s.geo, *L.wb[0], L.st[0]);
   if (rc == CYR_OK) {
@@ -200,7 +239,6 @@ cyr_status cyr_simulate_batch(uint32_t count, const cyr_sim_desc *descs, const v
   }
   const hipError_t se = hipStreamSynchronize(st);
   if (rc == CYR_OK && se != hipSuccess) rc = hip_fail(se, "hipStreamSynchronize", __FILE__, __LINE__);
-  L.destroy();
   free_buffer(s.tables);
   return rc;
 }

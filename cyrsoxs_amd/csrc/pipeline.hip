@@ -981,6 +981,7 @@ cyr_status cyr_release_workspace(void) {
   }
   hipSetDevice(cur);
   fft_release();
+  release_batch_buffers();
   return CYR_OK;
 }
 

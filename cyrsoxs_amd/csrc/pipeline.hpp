@@ -49,6 +49,9 @@ enum RunMode {
 uint32_t energy_lanes();
 void set_energy_lanes(uint32_t n);
 
+// Frees the per-device streams and work buffers of the batch executor.
+void release_batch_buffers();
+
 // Device bytes a simulation needs beyond its morphology and output.
 uint64_t work_bytes(const cyr_sim_desc *d, const SimGeometry &g);
 // Bytes of the device morphology in its storage format.

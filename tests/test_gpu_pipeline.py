@@ -58,6 +58,8 @@ CASES = {
     "thin2d": dict(dims=(64, 64, 1), M=3, morph=0, kw=dict(interpolation=1)),
     "thin2d_rotmask": dict(dims=(48, 48, 1), M=2, morph=1, kw=dict(interpolation=0, rot_mask=1)),
     "odd_grid": dict(dims=(30, 30, 10), M=2, morph=0, kw=dict(interpolation=1)),
+    "radix3_384": dict(dims=(384, 384, 4), M=2, morph=0, kw=dict(interpolation=1, windowing=1)),
+    "pow2_128": dict(dims=(128, 128, 6), M=2, morph=1, kw=dict(interpolation=0)),
 }
 
 
@@ -183,3 +185,23 @@ def test_autotune_records_a_choice():
         assert capi.get_tuning(c["dims"], capi.FP32) == (cols, target)
     finally:
         capi.set_tuning(c["dims"], capi.FP32, 0, 0)
+
+
+@pytest.mark.parametrize("prec,storage,tol", [(capi.FP64, capi.STORE_NATIVE, 1e-10), (capi.FP32, capi.STORE_NATIVE, 1e-4),
+                                              (capi.FP32, capi.STORE_FP16, 2e-2), (capi.FP32, capi.STORE_BF16, 1e-1)])
+def test_thinfilm_precision(prec, storage, tol):
+    """The thin-film morphology (3 phases, in-plane director) at 256^2 x 1 with the
+    full 0:2:358 rotation against the FP64 oracle; FP16/BF16 morphology storage
+    carries its own (reported) error."""
+    n = 256
+    vox = synthetic.thinfilm_morphology(n, seed=3).numpy().astype(np.float64)
+    energies = [284.0, 286.5]
+    mats = synthetic.lorentzian_constants(energies, 3, seed=3)
+    angles = (0.0, 2.0, 358.0)
+    ref = _oracle("thinfilm", O.Sim((n, n, 1), 3, 1.0, energies, angles, interpolation=1), vox, mats)
+    sim = capi.Sim((n, n, 1), 3, 1.0, energies, angles, interpolation=1, precision=prec, storage=storage,
+                   pipeline=capi.PIPE_EBASIS)
+    got = capi.simulate(sim, vox, mats)
+    err, same = pattern_rel_l2(got, ref)
+    print(f"thin film prec={prec} storage={storage}: rel L2 {err:.3e}")
+    assert same and err < tol
