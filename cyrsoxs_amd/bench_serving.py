@@ -152,9 +152,12 @@ def main(argv=None):
         res = serve(r, a.window, a.slots, capi.PIPE_EBASIS, a.seed * 1000 + i)
         print(json.dumps(res), file=sys.stderr, flush=True)
         results.append(res)
-    fifo = serve(a.fifo_rate, a.window, 1, capi.PIPE_EBASIS, a.seed * 1000 + 77)
+    # the same request stream as the continuous-batching run at that rate (or a
+    # fresh one if that rate was not run), served FIFO on one slot
+    fi = rates.index(a.fifo_rate) if a.fifo_rate in rates else 77
+    fifo = serve(a.fifo_rate, a.window, 1, capi.PIPE_EBASIS, a.seed * 1000 + fi)
     print(json.dumps(fifo), file=sys.stderr, flush=True)
-    plain = serve(min(rates), a.plain_window, 1, capi.PIPE_PLAIN, a.seed * 1000 + 99)
+    plain = serve(min(rates), a.plain_window, 1, capi.PIPE_PLAIN, a.seed * 1000 + rates.index(min(rates)))
     print(json.dumps(plain), file=sys.stderr, flush=True)
     out = {"metric": "p50/p99 latency and voxel·angle·energy/s under a Poisson request stream",
            "workload": "grids " + a.grids + ", 2-4 materials, 1-50 energies 280-305 eV, EAngle 0:5:175, linear, "
