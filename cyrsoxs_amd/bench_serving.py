@@ -89,15 +89,17 @@ def main(argv=None):
                                pipeline=capi.PIPE_EBASIS))
 
     def serve(rate, window, slots, pipeline, seed):
+        # arrival times and requests drawn in turn, so a shorter window serves a
+        # prefix of the same stream
         rng = np.random.default_rng(seed)
-        arrivals = []
+        arrivals, reqs = [], []
         t = 0.0
         while True:
             t += rng.exponential(1.0 / rate)
             if t > window:
                 break
             arrivals.append(t)
-        reqs = [make_request(rng, pipeline) for _ in arrivals]
+            reqs.append(make_request(rng, pipeline))
         ids, units = [], {}
         with capi.Engine(slots=slots) as eng:
             t_start = time.perf_counter()
@@ -130,7 +132,6 @@ def main(argv=None):
                 eng.wait(rid)
             t_end = time.perf_counter()
             lat, done_units, failed = [], 0.0, 0
-            last_done = 0.0
             for rid in ids:
                 done, st = eng.poll(rid)
                 if st == 0:
