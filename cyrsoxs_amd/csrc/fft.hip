@@ -24,6 +24,7 @@
 #include <vector>
 
 #include "fft.hpp"
+#include "fftpolicy.hpp"
 #include "stockham.hpp"
 
 namespace cyr {
@@ -146,6 +147,93 @@ __global__ void __launch_bounds__(1024) fft_pencil_kernel(StagePlan P, PassArgs 
         break;
     }
   }
+}
+
+// Compile-time codelet pass (lengths with a CtPlan): pencils held in registers,
+// stages exchanged through LDS with immediate offsets.  Contiguous pencils (X)
+// are padded LDS rows, several per wave when short, twiddles read from the
+// global table; strided pencils (Y, Z) are PP-column tiles whose twiddles are
+// staged in LDS next to the tile once per block.
+template <typename T, class F, bool CONTIG, int WG>
+__global__ void __launch_bounds__(WG) fft_ct_kernel(PassArgs A, const cx<T> *__restrict__ tw) {
+  constexpr int E = F::kE, TPP = F::TPP, N = TPP * E;
+  extern __shared__ unsigned char smem_raw[];
+  cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
+  const cx<T> *twp = tw;
+  uint32_t t, p;
+  if constexpr (CONTIG) {
+    t = threadIdx.x % TPP;
+    p = threadIdx.x / TPP;
+  } else {
+    cx<T> *ltw = lds + (size_t)A.pp * N;
+    for (uint32_t i = threadIdx.x; i < (uint32_t)N; i += WG) ltw[i] = tw[i];
+    lds_sync();
+    twp = ltw;
+    p = threadIdx.x % A.pp;
+    t = threadIdx.x / A.pp;
+  }
+  const uint32_t g = blockIdx.x % A.groups;
+  const uint32_t o = blockIdx.x / A.groups;
+  const uint32_t pencil = g * A.pp + p;
+  const bool active = pencil < A.inner_count;
+  cx<T> *gbase = reinterpret_cast<cx<T> *>(A.vol[blockIdx.y]) + (uint64_t)(active ? pencil : 0) * A.inner_stride +
+                 (uint64_t)o * A.outer_stride;
+  cx<T> v[E];
+#pragma unroll
+  for (int i = 0; i < E; i++) v[i] = active ? gbase[(uint64_t)F::in_pos(t, i) * A.elem_stride] : mk<T>(0, 0);
+  F::run(v, t, p, lds, twp);
+  if (active) {
+#pragma unroll
+    for (int i = 0; i < E; i++) gbase[(uint64_t)F::out_pos(t, i) * A.elem_stride] = v[i];
+  }
+}
+
+template <typename T, int N>
+hipError_t launch_ct_n(bool contig, uint32_t nc, PassArgs &A, hipStream_t stream, const cx<T> *tw) {
+  constexpr int E = N == 384 ? 12 : 8, TPP = N / E;
+  if (contig) {
+    using F = typename CtPlan<T, N, true, 1>::type;
+    constexpr int WG = TPP <= 64 ? 64 : TPP;
+    A.pp = WG / TPP;
+    A.tpp = TPP;
+    A.groups = (A.inner_count + A.pp - 1) / A.pp;
+    const uint64_t blocks = (uint64_t)A.groups * A.outer_count;
+    if (blocks > 0x7fffffffull) return hipErrorInvalidConfiguration;
+    const size_t lds = (size_t)A.pp * F::kPitch * sizeof(cx<T>);
+    hipLaunchKernelGGL((fft_ct_kernel<T, F, true, WG>), dim3((unsigned)blocks, nc), dim3(WG), lds, stream, A, tw);
+  } else {
+    constexpr int PP = N <= 512 ? 16 : (N == 1024 ? 8 : 4);
+    using F = typename CtPlan<T, N, false, PP>::type;
+    constexpr int WG = PP * TPP;
+    A.pp = PP;
+    A.tpp = TPP;
+    A.groups = (A.inner_count + PP - 1) / PP;
+    const uint64_t blocks = (uint64_t)A.groups * A.outer_count;
+    if (blocks > 0x7fffffffull) return hipErrorInvalidConfiguration;
+    const size_t lds = ((size_t)PP * N + N) * sizeof(cx<T>);
+    hipLaunchKernelGGL((fft_ct_kernel<T, F, false, WG>), dim3((unsigned)blocks, nc), dim3(WG), lds, stream, A, tw);
+  }
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_ct(uint32_t n, bool contig, uint32_t nc, PassArgs &A, hipStream_t stream, const cx<T> *tw,
+                     bool *found) {
+  *found = true;
+  switch (n) {
+    case 64: return launch_ct_n<T, 64>(contig, nc, A, stream, tw);
+    case 128: return launch_ct_n<T, 128>(contig, nc, A, stream, tw);
+    case 256: return launch_ct_n<T, 256>(contig, nc, A, stream, tw);
+    case 384:
+      if constexpr (sizeof(T) == 4) return launch_ct_n<T, 384>(contig, nc, A, stream, tw);
+      break;
+    case 512: return launch_ct_n<T, 512>(contig, nc, A, stream, tw);
+    case 1024: return launch_ct_n<T, 1024>(contig, nc, A, stream, tw);
+    case 2048: return launch_ct_n<T, 2048>(contig, nc, A, stream, tw);
+    default: break;
+  }
+  *found = false;
+  return hipSuccess;
 }
 
 // Direct O(n^2) DFT of each pencil for lengths with prime factors > 5: the
@@ -350,11 +438,30 @@ cyr_status fft_axis(void *const *vols, uint32_t count, uint32_t nx, uint32_t ny,
   }
   const Factoring f = factor(n);
   const size_t lds_max = max_lds_bytes();
+  // compile-time codelets unless a shape override asks for the run-time plan
+  bool use_ct = ct_length(n) && (sizeof(T) == 4 || ct_elements(n) <= 8);
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (g_shapes.count(std::make_tuple(n, (int)contig, prec_of<T>()))) use_ct = false;
+  }
+  if (use_ct) {
+    const uint32_t pp = contig ? 1 : (n <= 512 ? 16 : (n == 1024 ? 8 : 4));
+    const size_t need = contig ? (size_t)(64 / (n / ct_elements(n)) > 1 ? 64 / (n / ct_elements(n)) : 1) *
+                                     ct_row_pitch(n) * sizeof(cx<T>)
+                               : ((size_t)pp * n + n) * sizeof(cx<T>);
+    if (need > lds_max) use_ct = false;
+  }
   for (uint32_t c0 = 0; c0 < count; c0 += 8) {
     const uint32_t nc = count - c0 < 8 ? count - c0 : 8;
     for (uint32_t i = 0; i < nc; i++) A.vol[i] = vols[c0 + i];
     bool done = false;
-    if (f.ok) {
+    if (use_ct) {
+      bool found = false;
+      const hipError_t e = launch_ct<T>(n, contig, nc, A, stream, tw, &found);
+      if (e != hipSuccess) return hip_fail(e, "fft_ct_kernel", __FILE__, __LINE__);
+      done = found;
+    }
+    if (!done && f.ok) {
       // Elements per thread: the radix lcm, doubled while a pencil would need more
       // than 256 threads; pencils per block: up to 256 threads (contiguous) or a
       // 16-pencil coalescing group (strided), bounded by LDS.

@@ -16,7 +16,9 @@ pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
 
 FFT_SHAPES = [(64, 64, 64), (16, 12, 10), (384, 8, 3), (48, 48, 48), (128, 128, 32), (256, 256, 1),
-              (2048, 16, 1), (7, 11, 13), (30, 20, 5), (512, 4, 2)]
+              (2048, 16, 1), (7, 11, 13), (30, 20, 5), (512, 4, 2),
+              # compile-time codelets on strided axes with fewer pencils than a tile
+              (8, 8, 512), (6, 1024, 2), (5, 384, 3)]
 
 
 def _cplx(prec):
