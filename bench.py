@@ -83,6 +83,8 @@ def main(argv=None):
     ap.add_argument("--plain-steps", type=int, default=1)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--energies", type=int, default=0,
+                    help="run only the first N energies (profiling runs; 0 = all 20)")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--lanes", type=int, default=1,
                     help="(energy, k) units in flight in the timed steps (the per-kernel roofline pass uses 1)")
@@ -102,6 +104,8 @@ def main(argv=None):
 
     torch.cuda.set_device(0)
     desc = synthetic.config1_desc()
+    if a.energies:
+        desc["energies"] = desc["energies"][:a.energies]
     dims, M = desc["dims"], desc["num_materials"]
     nx, ny, nz = dims
     V = nx * ny * nz
@@ -117,10 +121,11 @@ def main(argv=None):
                         pipeline=pipeline)
 
     if a.copy_calib:
+        # a plain elementwise kernel of known traffic: reads 1 GiB, writes 1 GiB
         src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
         for _ in range(3):
-            dst.copy_(src)
+            torch.neg(src, out=dst)
         torch.cuda.synchronize()
         del src, dst
 

@@ -110,6 +110,17 @@ template <typename T> struct Vec3 { T x, y, z; };
 
 inline unsigned ceil_div(uint64_t a, uint64_t b) { return (unsigned)((a + b - 1) / b); }
 
+// XCD-aware block order.  The dispatcher hands work group w to XCD w mod 8 (MI355X
+// has 8 XCDs, each with its own L2); kernels whose neighbouring tiles share HBM
+// sectors launch a 1D grid padded to a multiple of 8 and map block w to tile
+//   q = (w mod 8) * (grid / 8) + w / 8,
+// so every XCD walks one contiguous range of tiles and neighbours meet in its L2.
+constexpr unsigned kXcds = 8;
+inline unsigned xcd_grid(uint64_t tiles) { return ceil_div(tiles, kXcds) * kXcds; }
+__device__ inline uint32_t xcd_tile(uint32_t w, uint32_t grid) {
+  return (w % kXcds) * (grid / kXcds) + w / kXcds;
+}
+
 // ---------------------------------------------------------------------------
 // Per-device cached work buffers (grown on demand, freed by
 // cyr_release_workspace or at process exit).

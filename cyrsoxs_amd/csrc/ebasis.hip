@@ -193,10 +193,14 @@ __global__ void __launch_bounds__(WG) ebasis_yz_kernel(StagePlan P, YzArgs<T> a)
   cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
   const F fft(P, a.tpp, a.pp, false);
   const uint32_t pp = a.pp, tpp = a.tpp;
+  // tile q = (z chunk, field, column group), column group fastest, in XCD order
+  const uint32_t groups = (a.nx + pp - 1) / pp;
+  const uint32_t q = xcd_tile(blockIdx.x, gridDim.x);
+  if (q >= groups * kEbFields * a.nchunks) return;  // grid padding: the whole block leaves
+  const uint32_t grp = q % groups, c = (q / groups) % kEbFields, chunk = q / (groups * kEbFields);
   const uint32_t p = threadIdx.x % pp, t = threadIdx.x / pp;
-  const uint32_t kx = blockIdx.x * pp + p;
+  const uint32_t kx = grp * pp + p;
   const bool active = kx < a.nx;
-  const uint32_t c = blockIdx.y;
   const uint64_t P2 = (uint64_t)a.nx * a.ny;
   const cx<T> *vol = a.vol[c] + (active ? kx : 0);
 
@@ -220,7 +224,7 @@ __global__ void __launch_bounds__(WG) ebasis_yz_kernel(StagePlan P, YzArgs<T> a)
     cx<T> *ltw = lty + a.ny;
     for (uint32_t i = threadIdx.x; i < a.ny; i += blockDim.x) lty[i] = a.twy[i];
     for (uint32_t i = threadIdx.x; i < a.nz; i += blockDim.x) ltw[i] = a.twz[i];
-    const uint32_t zb = blockIdx.z * a.zchunk;
+    const uint32_t zb = chunk * a.zchunk;
     const uint32_t ze = zb + a.zchunk < a.nz ? zb + a.zchunk : a.nz;
     // per element: (z-bin << 16) | running exponent z * bin mod Nz; 0xffffffff = no bin
     uint32_t kx16[E];
@@ -274,8 +278,8 @@ __global__ void __launch_bounds__(WG) ebasis_yz_kernel(StagePlan P, YzArgs<T> a)
 #pragma unroll
       for (int i = 0; i < E; i++) {
         const uint64_t o = (uint64_t)fft.out_pos(t, i) * a.nx + kx;
-        a.part[(((uint64_t)blockIdx.z * NG + 0) * kEbFields + c) * P2 + o] = acc0[i];
-        if constexpr (NG == 2) a.part[(((uint64_t)blockIdx.z * NG + 1) * kEbFields + c) * P2 + o] = acc1[i];
+        a.part[(((uint64_t)chunk * NG + 0) * kEbFields + c) * P2 + o] = acc0[i];
+        if constexpr (NG == 2) a.part[(((uint64_t)chunk * NG + 1) * kEbFields + c) * P2 + o] = acc1[i];
       }
     }
   }
@@ -755,7 +759,7 @@ template <typename T> cyr_status launch_ebasis_yz(YzArgs<T> a, int ng, hipStream
   a.twy = twiddles<T>(a.ny, &s);
   if (!a.twy) return s;
   const uint32_t groups = ceil_div(a.nx, a.pp);
-  const dim3 grid(groups, kEbFields, a.nchunks), block(a.pp * a.tpp);
+  const dim3 grid(xcd_grid((uint64_t)groups * kEbFields * a.nchunks)), block(a.pp * a.tpp);
   const size_t lds =
       (size_t)a.pp * a.ny * sizeof(cx<T>) + (three ? ((size_t)a.nz + a.ny) * sizeof(cx<T>) : 0);
   if (lds > device_max_lds()) return fail(CYR_E_UNSUPPORTED, "E-basis: Y tile does not fit in LDS");

@@ -33,6 +33,7 @@ namespace {
 
 struct PassArgs {
   void *vol[8];
+  uint32_t nvol;  // volumes of this launch (1D XCD-ordered grids)
   uint64_t elem_stride;
   uint64_t inner_stride;
   uint64_t outer_stride;
@@ -165,6 +166,8 @@ __global__ void __launch_bounds__(WG) fft_ct_kernel(PassArgs A, const cx<T> *__r
     t = threadIdx.x % TPP;
     p = threadIdx.x / TPP;
   } else {
+    if (xcd_tile(blockIdx.x, gridDim.x) >= A.groups * A.outer_count * A.nvol)
+      return;  // grid padding: the whole block leaves before any barrier
     cx<T> *ltw = lds + (size_t)A.pp * N;
     for (uint32_t i = threadIdx.x; i < (uint32_t)N; i += WG) ltw[i] = tw[i];
     lds_sync();
@@ -172,11 +175,19 @@ __global__ void __launch_bounds__(WG) fft_ct_kernel(PassArgs A, const cx<T> *__r
     p = threadIdx.x % A.pp;
     t = threadIdx.x / A.pp;
   }
-  const uint32_t g = blockIdx.x % A.groups;
-  const uint32_t o = blockIdx.x / A.groups;
+  uint32_t tile = blockIdx.x, vol = blockIdx.y;
+  if constexpr (!CONTIG) {
+    // 1D grid in XCD order: neighbouring column groups (sharing HBM sectors) on one XCD
+    const uint32_t q = xcd_tile(blockIdx.x, gridDim.x);
+    const uint32_t per_vol = A.groups * A.outer_count;
+    vol = q / per_vol;
+    tile = q % per_vol;
+  }
+  const uint32_t g = tile % A.groups;
+  const uint32_t o = tile / A.groups;
   const uint32_t pencil = g * A.pp + p;
   const bool active = pencil < A.inner_count;
-  cx<T> *gbase = reinterpret_cast<cx<T> *>(A.vol[blockIdx.y]) + (uint64_t)(active ? pencil : 0) * A.inner_stride +
+  cx<T> *gbase = reinterpret_cast<cx<T> *>(A.vol[vol]) + (uint64_t)(active ? pencil : 0) * A.inner_stride +
                  (uint64_t)o * A.outer_stride;
   cx<T> v[E];
 #pragma unroll
@@ -208,10 +219,11 @@ hipError_t launch_ct_n(bool contig, uint32_t nc, PassArgs &A, hipStream_t stream
     A.pp = PP;
     A.tpp = TPP;
     A.groups = (A.inner_count + PP - 1) / PP;
-    const uint64_t blocks = (uint64_t)A.groups * A.outer_count;
-    if (blocks > 0x7fffffffull) return hipErrorInvalidConfiguration;
+    A.nvol = nc;
+    const uint64_t tiles = (uint64_t)A.groups * A.outer_count * nc;
+    if (tiles > 0x7ffffff0ull) return hipErrorInvalidConfiguration;
     const size_t lds = ((size_t)PP * N + N) * sizeof(cx<T>);
-    hipLaunchKernelGGL((fft_ct_kernel<T, F, false, WG>), dim3((unsigned)blocks, nc), dim3(WG), lds, stream, A, tw);
+    hipLaunchKernelGGL((fft_ct_kernel<T, F, false, WG>), dim3(xcd_grid(tiles)), dim3(WG), lds, stream, A, tw);
   }
   return hipGetLastError();
 }
