@@ -7,8 +7,10 @@ uniform Euler angles; 20 energies (280-299 eV) x 36 E angles (0:5:175), linear
 Ewald interpolation, Hanning window, FP32.  A step is one whole simulation
 (every energy and angle) with the morphology already resident in HBM.
 
-value = voxel*angle*energy per second of the production (E-basis) pipeline; the
-plain per-angle pipeline (the reference's structure) is timed beside it.  The
+value = voxel*angle*energy per second of the production pipeline (--pipeline,
+default the material-basis executor); the other fused executor (E-basis) and the
+plain per-angle pipeline (the reference's structure) are timed beside it and
+their patterns compared with the headline's.  The
 roofline object reports the dominant kernel's achieved HBM bandwidth: every
 launch of the timed steps is bracketed by HIP events on the library stream it
 runs on (cyr_kernel_timing), and its algorithmic bytes (inputs read once,
@@ -31,6 +33,15 @@ HBM_PEAK_GBS = 8000.0
 # workload, scaled by the counters' reading for a copy kernel of known size
 # (profiles/ holds the runs); None where not measured.
 TRAFFIC = {}
+# timing labels of the library (cyr_kernel_timing) -> kernel symbol
+KERNEL_OF = {"xfields": "xfields_kernel", "fft_y": "fft_ct_kernel (Y pass)", "fft_z": "fft_ct_kernel (Z pass)",
+             "mbasis_extract": "mbasis_extract_kernel", "mbasis_angles": "mbasis_angles_kernel",
+             "mbasis_bands": "mbasis_bands_kernel", "ebasis_yz": "ebasis_yz_kernel", "ebasis_dc": "ebasis_dc_kernel",
+             "ebasis_angles": "ebasis_angles_kernel", "ebasis_bins": "ebasis_bins_kernel",
+             "rotate_all": "rotate_all_kernel"}
+PIPE_DESC = {"mbasis": "mbasis (7 energy-independent basis fields per material, packed in pairs: X FFT fused with "
+                       "the fields, Y and Z FFT passes, values in the Ewald band; per energy a 2D combination)",
+             "ebasis": "ebasis (5 tensor fields per energy, XY FFT, pruned z DFT at the Ewald bins)"}
 
 
 def _pct(xs, p):
@@ -82,6 +93,9 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--plain-steps", type=int, default=1)
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--pipeline", choices=("mbasis", "ebasis"), default="mbasis")
+    ap.add_argument("--other-steps", type=int, default=2,
+                    help="timed steps of the other fused executor (0 = skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--energies", type=int, default=0,
                     help="run only the first N energies (profiling runs; 0 = all 20)")
@@ -129,7 +143,9 @@ def main(argv=None):
         torch.cuda.synchronize()
         del src, dst
 
-    sim = mk(capi.PIPE_EBASIS)
+    pipes = {"mbasis": capi.PIPE_MBASIS, "ebasis": capi.PIPE_EBASIS}
+    other = "ebasis" if a.pipeline == "mbasis" else "mbasis"
+    sim = mk(pipes[a.pipeline])
     nA = sim.num_angles
     units = V * nA * nE
     out = torch.empty((nE, 1, ny, nx), dtype=torch.float32, device="cuda")
@@ -139,11 +155,11 @@ def main(argv=None):
 
     # one-time per grid shape, like an FFT plan: pick the Y/z launch shape on this device
     t0 = time.perf_counter()
-    tuned = capi.autotune(sim)
+    tuned = capi.autotune(mk(capi.PIPE_EBASIS))
     tune_s = time.perf_counter() - t0
-    log(f"autotune: columns {tuned[0]}, target blocks {tuned[1]} ({tune_s:.2f} s)")
+    log(f"autotune (E-basis Y/z): columns {tuned[0]}, target blocks {tuned[1]} ({tune_s:.2f} s)")
     capi.set_energy_lanes(a.lanes)
-    log(f"E-basis pipeline ({a.lanes} lane(s)): warmup {a.warmup}, steps {a.steps}")
+    log(f"{a.pipeline} pipeline ({a.lanes} lane(s)): warmup {a.warmup}, steps {a.steps}")
     for _ in range(a.warmup):
         run(sim, out)
     torch.cuda.synchronize()
@@ -156,7 +172,7 @@ def main(argv=None):
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     value = units * a.steps / elapsed
-    log(f"E-basis: {1000.0 * elapsed / a.steps:.2f} ms/step")
+    log(f"{a.pipeline}: {1000.0 * elapsed / a.steps:.2f} ms/step")
 
     # per-kernel roofline pass: one unit in flight, every launch bracketed by HIP
     # events on the stream it is queued on (the library's stream)
@@ -175,12 +191,35 @@ def main(argv=None):
         "data": "synthetic (seeded Gaussian-filtered bulk morphology, Lorentzian optical constants)",
         "config": {"workload": "bulk 512x512x128, 4 materials, PhysSize 2 nm, 20 energies x 36 E angles "
                                "(0:5:175), linear Ewald, Hanning, FP32",
-                   "pipeline": "ebasis (5 tensor fields/energy, XY FFT, pruned z DFT at the Ewald bins)",
+                   "pipeline": PIPE_DESC[a.pipeline],
                    "lanes": a.lanes,
                    "units_per_step": units,
                    "yz_tuning": {"columns": tuned[0], "target_blocks": tuned[1], "autotune_s": round(tune_s, 3)}},
         "latency_ms": {"p50": 1000.0 * _pct(lat, 50), "p99": 1000.0 * _pct(lat, 99)},
     }
+
+    def compare(g, p):
+        fin = np.isfinite(p)
+        rel = [float(np.linalg.norm((g[e][fin[e]] - p[e][fin[e]])) / np.linalg.norm(p[e][fin[e]]))
+               for e in range(nE)]
+        return max(rel), bool(np.array_equal(np.isnan(g), np.isnan(p)))
+
+    if a.other_steps > 0:
+        osim = mk(pipes[other])
+        oout = torch.empty_like(out)
+        log(f"{other} pipeline: warmup 1, steps {a.other_steps}")
+        run(osim, oout)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(a.other_steps):
+            run(osim, oout)
+        torch.cuda.synchronize()
+        oel = time.perf_counter() - t0
+        rel, same = compare(out.double().cpu().numpy(), oout.double().cpu().numpy())
+        result[other] = {"value": units * a.other_steps / oel, "ms_per_step": 1000.0 * oel / a.other_steps,
+                         "steps": a.other_steps, "structure": PIPE_DESC[other],
+                         f"{a.pipeline}_vs_{other}_max_rel_l2": rel, "nan_mask_identical": same}
+        del oout
 
     if not a.no_plain:
         psim = mk(capi.PIPE_PLAIN)
@@ -193,17 +232,12 @@ def main(argv=None):
             run(psim, pout)
         torch.cuda.synchronize()
         pel = time.perf_counter() - t0
-        g = out.double().cpu().numpy()
-        p = pout.double().cpu().numpy()
-        fin = np.isfinite(p)
-        rel = [float(np.linalg.norm((g[e][fin[e]] - p[e][fin[e]])) / np.linalg.norm(p[e][fin[e]]))
-               for e in range(nE)]
+        rel, same = compare(out.double().cpu().numpy(), pout.double().cpu().numpy())
         result["plain"] = {"value": units * a.plain_steps / pel, "ms_per_step": 1000.0 * pel / a.plain_steps,
                            "steps": a.plain_steps,
                            "structure": "per angle: polarization, 3 full 3D FFTs, DC, Ewald, rotate/accumulate",
-                           "speedup_ebasis_vs_plain": value / (units * a.plain_steps / pel),
-                           "ebasis_vs_plain_max_rel_l2": max(rel),
-                           "nan_mask_identical": bool(np.array_equal(np.isnan(g), np.isnan(p)))}
+                           f"speedup_{a.pipeline}_vs_plain": value / (units * a.plain_steps / pel),
+                           f"{a.pipeline}_vs_plain_max_rel_l2": rel, "nan_mask_identical": same}
 
     # ---- roofline of the dominant kernel (largest share of the timed region)
     tot = sum(k["total_us"] for k in kern.values())
@@ -214,7 +248,7 @@ def main(argv=None):
                        "algorithmic_bytes": k["avg_bytes"], "GBs": round(gbs, 1)}
     dom = max(kern, key=lambda n: kern[n]["total_us"])
     ach = table[dom]["GBs"]
-    result["roofline"] = {"bound": "hbm", "kernel": f"{dom}_kernel", "achieved": ach, "peak": HBM_PEAK_GBS,
+    result["roofline"] = {"bound": "hbm", "kernel": KERNEL_OF.get(dom, dom), "achieved": ach, "peak": HBM_PEAK_GBS,
                           "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": TRAFFIC.get(dom),
                           "algorithmic_bytes": kern[dom]["avg_bytes"], "avg_us": kern[dom]["avg_us"],
                           "kernels": table}

@@ -21,7 +21,7 @@ STATUS = {0: "CYR_OK", 1: "CYR_E_INVALID", 2: "CYR_E_NOMEM", 3: "CYR_E_DEVICE", 
           9: "CYR_E_NOTFOUND"}
 FP32, FP64 = 0, 1
 STORE_NATIVE, STORE_FP16, STORE_BF16 = 0, 1, 2
-PIPE_PLAIN, PIPE_EBASIS = 0, 1
+PIPE_PLAIN, PIPE_EBASIS, PIPE_MBASIS = 0, 1, 2
 
 _dp = ctypes.POINTER(ctypes.c_double)
 

@@ -18,8 +18,9 @@
 // Four launches per (energy, k):
 //   ebasis_bins_kernel   pixel -> FFT z bin of its (kx, ky) column (FP64 geometry);
 //                        with linear interpolation the second bin is bin - 1 (mod Nz)
-//   ebasis_polx_kernel   the five fields evaluated at the X FFT's Stockham stage-0
-//                        positions, X FFT in LDS, X-transformed fields Gx written once
+//   xfields_kernel       (xfields.hpp, tensor mode) the five fields evaluated at the
+//                        X FFT's Stockham stage-0 positions, X FFT in LDS,
+//                        X-transformed fields Gx written once
 //   ebasis_yz_kernel     per z slice: Y FFT of a group of x columns, each output
 //                        (kx, ky) accumulated into its column's z bin(s):
 //                        F(kx,ky,kz) = sum_z Gy(kx,ky,z) W^(z kz); z is split into
@@ -43,6 +44,7 @@
 #include "fftpolicy.hpp"
 #include "stockham.hpp"
 #include "tensor.hpp"
+#include "xfields.hpp"
 
 namespace cyr {
 
@@ -63,124 +65,6 @@ __global__ void __launch_bounds__(256) ebasis_bins_kernel(EwaldGeom g, int32_t *
   const int kind = ewald_geometry(g, X, Y, &z0, &f);
   const uint64_t col = (uint64_t)unshift(Y, g.ny) * g.nx + unshift(X, g.nx);
   kzt[col] = kind == 0 ? -1 : (int32_t)(g.enable2d ? 0 : unshift(z0, g.nz));
-}
-
-// ---- the five tensor fields of one voxel -------------------------------------
-template <typename T, typename S, int MORPH>
-__device__ inline void tensor_fields(const S *__restrict__ vox, const MatConst<T> *__restrict__ mats, uint32_t M,
-                                     uint64_t V, uint64_t idx, const Mat3<T> &Q, bool rot, T fr[kEbFields],
-                                     T fi[kEbFields]) {
-#pragma unroll
-  for (int j = 0; j < kEbFields; j++) fr[j] = fi[j] = 0;
-  for (uint32_t m = 0; m < M; m++) {
-    const MatConst<T> mc = mats[m];
-    T v[4];
-    VoxLoad<T, S>::get(vox, (uint64_t)m * V + idx, v);
-    T sx, sy, sz, pa, isor, isoi;
-    if (MORPH == 1) {
-      sx = v[0];
-      sy = v[1];
-      sz = v[2];
-      pa = (T)1;
-      const T ss = sx * sx + sy * sy + sz * sz;
-      isor = mc.br * ss + mc.cr * v[3];
-      isoi = mc.bi * ss + mc.ci * v[3];
-    } else {
-      T st, ct, sp, cp;
-      sincos_t(v[1], &st, &ct);
-      sincos_t(v[2], &sp, &cp);
-      sx = cp * st;
-      sy = sp * st;
-      sz = ct;
-      pa = v[3] * v[0];
-      const T pu = v[3] - pa;
-      isor = mc.br * pa + mc.cr * pu;
-      isoi = mc.bi * pa + mc.ci * pu;
-    }
-    if (rot) {  // s' = Q^T s
-      const T a = Q.m[0] * sx + Q.m[3] * sy + Q.m[6] * sz;
-      const T b = Q.m[1] * sx + Q.m[4] * sy + Q.m[7] * sz;
-      const T c = Q.m[2] * sx + Q.m[5] * sy + Q.m[8] * sz;
-      sx = a;
-      sy = b;
-      sz = c;
-    }
-    const T tx = pa * sx, ty = pa * sy;
-    const T q00 = tx * sx, q01 = tx * sy, q02 = tx * sz, q11 = ty * sy, q12 = ty * sz;
-    fr[0] += isor + mc.dr * q00;
-    fi[0] += isoi + mc.di * q00;
-    fr[1] += mc.dr * q01;
-    fi[1] += mc.di * q01;
-    fr[2] += mc.dr * q02;
-    fi[2] += mc.di * q02;
-    fr[3] += isor + mc.dr * q11;
-    fi[3] += isoi + mc.di * q11;
-    fr[4] += mc.dr * q12;
-    fi[4] += mc.di * q12;
-  }
-}
-
-// ---- tensor fields + X FFT ------------------------------------------------------
-// A block holds pp rows of tpp threads (F::kE elements each); WG = 64 makes
-// every row exchange wave-local (the barriers compile to LDS waits).  LDS: the
-// pp x row tile.  The X twiddles are read from the global table: with many
-// independent single-wave blocks per CU their latency hides, while staging them
-// per block cost more than it saved (as did looping a block over several rows,
-// which spills registers).
-template <typename T, typename S, class F, int WG>
-__global__ void __launch_bounds__(WG, F::kE <= 8 ? 4 : 2)
-    ebasis_polx_kernel(StagePlan P, PolxArgs<T> a, const S *__restrict__ vox, int morph, int hann) {
-  constexpr int E = F::kE;
-  extern __shared__ unsigned char smem_raw[];
-  cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
-  const F fft(P, a.tpp, a.nx, true);
-  const uint32_t tpp = a.tpp;
-  const uint32_t t = threadIdx.x % tpp, p = threadIdx.x / tpp;
-  const uint32_t row = blockIdx.x * a.pp + p;  // row = y + ny z
-  const bool active = row < a.rows;
-  const uint32_t y = active ? row % a.ny : 0, z = active ? row / a.ny : 0;
-  const T k4 = (T)(1.0 / (4.0 * M_PI));
-  const T wyz = hann ? a.hy[y] * a.hz[z] * k4 : k4;
-  const bool rot = a.rot != 0;
-  T fre[kEbFields][E], fim[kEbFields][E];
-#pragma unroll
-  for (int i = 0; i < E; i++) {
-    const uint32_t x = fft.in_pos(t, i);
-    T fr[kEbFields], fi[kEbFields];
-    if (active) {
-      const uint64_t idx = (uint64_t)row * a.nx + x;
-      if (morph == 1) tensor_fields<T, S, 1>(vox, a.mats, a.M, a.V, idx, a.Q, rot, fr, fi);
-      else tensor_fields<T, S, 0>(vox, a.mats, a.M, a.V, idx, a.Q, rot, fr, fi);
-      const T w = hann ? a.hx[x] * wyz : wyz;
-#pragma unroll
-      for (int c = 0; c < kEbFields; c++) {
-        fr[c] *= w;
-        fi[c] *= w;
-      }
-    } else {
-#pragma unroll
-      for (int c = 0; c < kEbFields; c++) fr[c] = fi[c] = 0;
-    }
-#pragma unroll
-    for (int c = 0; c < kEbFields; c++) {
-      fre[c][i] = fr[c];
-      fim[c][i] = fi[c];
-    }
-  }
-#pragma unroll
-  for (int c = 0; c < kEbFields; c++) {
-    cx<T> v[E];
-#pragma unroll
-    for (int i = 0; i < E; i++) v[i] = mk<T>(fre[c][i], fim[c][i]);
-    fft.run(v, t, p, lds, a.twx);
-    if (active) {
-      cx<T> *o = a.out[c] + (uint64_t)row * a.nx;
-#pragma unroll
-      for (int i = 0; i < E; i++) o[fft.out_pos(t, i)] = v[i];
-    }
-    // (run() ends with a barrier after its last LDS reads, so the next field's
-    // first LDS store cannot overtake them)
-  }
 }
 
 // ---- Y FFT + z bins -----------------------------------------------------------
@@ -421,80 +305,13 @@ __global__ void __launch_bounds__(256) ebasis_angles_kernel(AnglesArgs<T> a) {
   }
 }
 
-// Elements-per-thread instantiations: FP32 up to 24 (radix lcm x 2^k), FP64 up to 8.
-template <typename T> constexpr bool e_allowed(int E) { return sizeof(T) == 4 || E <= 8; }
-
-// Launch shapes of the compile-time plans: X rows of <= 64 threads share a wave;
-// Y tiles are PP columns wide (16 columns = 128 B rows of complex64).
-constexpr int ct_polx_wg(int N, int E) { return N / E <= 64 ? 64 : N / E; }
+// Launch shapes of the compile-time Y plans: tiles are PP columns wide (16
+// columns = 128 B rows of complex64).
 constexpr int ct_yz_pp(int N) { return N <= 512 ? 16 : (N == 1024 ? 8 : 4); }
 uint32_t ct_yz_pp_of(uint32_t n) { return n <= 512 ? 16u : (n == 1024 ? 8u : 4u); }
 // Column-tile widths compiled for the compile-time Y plans (PP x tpp <= 1024).
 bool ct_yz_pp_ok(uint32_t n, uint32_t pp) {
   return (pp == 4 || pp == 8 || pp == 16) && pp * (n / ct_elements(n)) <= 1024;
-}
-
-template <typename T, typename S, int E>
-hipError_t launch_polx_rt(dim3 grid, dim3 block, size_t lds, hipStream_t st, const StagePlan &P,
-                          const PolxArgs<T> &a, const void *vox, int morph, int hann) {
-  hipLaunchKernelGGL((ebasis_polx_kernel<T, S, RtFft<T, E>, 256>), grid, block, lds, st, P, a, (const S *)vox,
-                     morph, hann);
-  return hipGetLastError();
-}
-
-template <typename T, typename S>
-hipError_t launch_polx_rt_e(uint32_t E, dim3 grid, dim3 block, size_t lds, hipStream_t st, const StagePlan &P,
-                            const PolxArgs<T> &a, const void *vox, int morph, int hann, bool *found) {
-  *found = true;
-#define CYR_PX(EV)                                                                      \
-  case EV:                                                                              \
-    if constexpr (e_allowed<T>(EV)) return launch_polx_rt<T, S, EV>(grid, block, lds, st, P, a, vox, morph, hann); \
-    break;
-  switch (E) {
-    CYR_PX(2)
-    CYR_PX(3)
-    CYR_PX(4)
-    CYR_PX(5)
-    CYR_PX(6)
-    CYR_PX(8)
-    CYR_PX(12)
-    CYR_PX(16)
-    CYR_PX(24)
-    default:
-      break;
-  }
-#undef CYR_PX
-  *found = false;
-  return hipSuccess;
-}
-
-template <typename T, typename S, int N>
-hipError_t launch_polx_ct(dim3 grid, size_t lds, hipStream_t st, const StagePlan &P, const PolxArgs<T> &a,
-                          const void *vox, int morph, int hann) {
-  using F = typename CtPlan<T, N, true, 1>::type;
-  constexpr int WG = ct_polx_wg(N, F::kE);
-  hipLaunchKernelGGL((ebasis_polx_kernel<T, S, F, WG>), grid, dim3(WG), lds, st, P, a, (const S *)vox, morph, hann);
-  return hipGetLastError();
-}
-
-template <typename T, typename S>
-hipError_t launch_polx_ct_n(uint32_t n, dim3 grid, size_t lds, hipStream_t st, const StagePlan &P,
-                            const PolxArgs<T> &a, const void *vox, int morph, int hann, bool *found) {
-  *found = true;
-  switch (n) {
-    case 64: return launch_polx_ct<T, S, 64>(grid, lds, st, P, a, vox, morph, hann);
-    case 128: return launch_polx_ct<T, S, 128>(grid, lds, st, P, a, vox, morph, hann);
-    case 256: return launch_polx_ct<T, S, 256>(grid, lds, st, P, a, vox, morph, hann);
-    case 384:
-      if constexpr (e_allowed<T>(12)) return launch_polx_ct<T, S, 384>(grid, lds, st, P, a, vox, morph, hann);
-      break;
-    case 512: return launch_polx_ct<T, S, 512>(grid, lds, st, P, a, vox, morph, hann);
-    case 1024: return launch_polx_ct<T, S, 1024>(grid, lds, st, P, a, vox, morph, hann);
-    case 2048: return launch_polx_ct<T, S, 2048>(grid, lds, st, P, a, vox, morph, hann);
-    default: break;
-  }
-  *found = false;
-  return hipSuccess;
 }
 
 template <int E> constexpr int rt_yz_wg() { return E <= 8 ? 1024 : 512; }
@@ -569,14 +386,6 @@ hipError_t launch_yz_ct_n(uint32_t n, dim3 grid, size_t lds, hipStream_t st, con
   return hipSuccess;
 }
 
-// Elements per thread for a pencil of length n: the radix lcm, doubled while the
-// pencil would need more than `tpp_cap` threads.
-uint32_t choose_e(uint32_t n, uint32_t lcm, uint32_t tpp_cap, uint32_t e_cap) {
-  uint32_t E = lcm;
-  while (n / E > tpp_cap && E * 2 <= e_cap && n % (E * 2) == 0) E *= 2;
-  return E;
-}
-
 // z chunks of the Y/z kernel: enough blocks (x groups x fields x chunks) to
 // reach target_blocks, at most one chunk per z slice.
 uint32_t yz_chunks(uint32_t nx, uint32_t nz, uint32_t pp, uint32_t target_blocks) {
@@ -587,20 +396,6 @@ uint32_t yz_chunks(uint32_t nx, uint32_t nz, uint32_t pp, uint32_t target_blocks
 }
 
 uint32_t yz_wg_of(uint32_t E) { return E <= 8 ? 1024u : 512u; }
-
-// Elements-per-thread values with a compiled kernel (launch_polx_rt_e / launch_yz_rt_e).
-bool e_instantiated(uint32_t E, int precision) {
-  switch (E) {
-    case 2: case 3: case 4: case 5: case 6: case 8: return true;
-    case 12: case 16: case 24: return precision != CYR_FP64;
-    default: return false;
-  }
-}
-
-// The compile-time plan applies (its elements per thread are compiled for T).
-bool ct_usable(uint32_t n, int precision) {
-  return ct_length(n) && (precision != CYR_FP64 || ct_elements(n) <= 8);
-}
 
 }  // namespace
 
@@ -660,44 +455,7 @@ cyr_status launch_ebasis_bins(const EwaldGeom &g, int32_t *kzt, hipStream_t st) 
 
 template <typename T>
 cyr_status launch_ebasis_polx(PolxArgs<T> a, const void *vox, int storage, int morph, bool hann, hipStream_t st) {
-  StagePlan P;
-  uint32_t lcm;
-  if (!make_stage_plan(a.nx, P, lcm)) return fail(CYR_E_UNSUPPORTED, "E-basis: Nx must be 2^a 3^b 5^c");
-  const int prec = sizeof(T) == 8 ? CYR_FP64 : CYR_FP32;
-  const bool ct = ct_usable(a.nx, prec);
-  const uint32_t E = ct ? ct_elements(a.nx) : choose_e(a.nx, lcm, 256, sizeof(T) == 8 ? 8 : 16);
-  a.tpp = a.nx / E;
-  if (a.tpp > 256) return fail(CYR_E_UNSUPPORTED, "E-basis: X row needs more than 256 threads");
-  // compile-time plans: rows of <= 64 threads share one wave; the run-time plan
-  // packs rows into 256-thread blocks
-  const uint32_t wg = ct ? (a.tpp <= 64 ? 64u : a.tpp) : 256u;
-  a.pp = wg / a.tpp;
-  if (a.pp < 1) a.pp = 1;
-  a.pitch = ct ? ct_row_pitch(a.nx) : a.nx;
-  const size_t lds = (size_t)a.pp * a.pitch * sizeof(cx<T>);
-  if (lds > device_max_lds()) return fail(CYR_E_UNSUPPORTED, "E-basis: row does not fit in LDS");
-  cyr_status s;
-  a.twx = twiddles<T>(a.nx, &s);
-  if (!a.twx) return s;
-  const dim3 grid(ceil_div(a.rows, a.pp)), block(a.pp * a.tpp);
-  bool found = false;
-  hipError_t e = hipSuccess;
-  auto go = [&](auto tag) {
-    using S = decltype(tag);
-    if (ct) e = launch_polx_ct_n<T, S>(a.nx, grid, lds, st, P, a, vox, morph, hann ? 1 : 0, &found);
-    else e = launch_polx_rt_e<T, S>(E, grid, block, lds, st, P, a, vox, morph, hann ? 1 : 0, &found);
-  };
-  if constexpr (sizeof(T) == 4) {
-    if (storage == CYR_STORE_FP16) go(half4{});
-    else if (storage == CYR_STORE_BF16) go(bf164{});
-    else go(float4{});
-  } else {
-    if (storage != CYR_STORE_NATIVE) return fail(CYR_E_UNSUPPORTED, "E-basis FP64: native morphology storage only");
-    go(double4{});
-  }
-  if (e != hipSuccess) return hip_fail(e, "ebasis_polx_kernel", __FILE__, __LINE__);
-  if (!found) return fail(CYR_E_UNSUPPORTED, "E-basis: no X kernel for this length");
-  return CYR_OK;
+  return launch_xfields<T, kXTensor>(a, vox, storage, morph, hann, st);
 }
 
 template <typename T> cyr_status ebasis_yz_plan(YzArgs<T> &a, uint32_t target_blocks) {
@@ -781,7 +539,9 @@ template <typename T> cyr_status launch_ebasis_yz(YzArgs<T> a, int ng, hipStream
 
 template <typename T> cyr_status launch_ebasis_dc(const DcArgs<T> &a, hipStream_t st) {
   if (a.nx < 2 || a.ny < 2 || !a.twy || (a.nz > 1 && !a.twz)) return fail(CYR_E_INVALID, "E-basis: DC tables missing");
-  hipLaunchKernelGGL((ebasis_dc_kernel<T>), dim3(a.nz, kEbFields), dim3(256), 0, st, a);
+  const uint32_t n = a.count ? a.count : kEbFields;
+  if (n > (uint32_t)kXMaxOut) return fail(CYR_E_INVALID, "DC sums: too many volumes");
+  hipLaunchKernelGGL((ebasis_dc_kernel<T>), dim3(a.nz, n), dim3(256), 0, st, a);
   CYR_LAUNCHED();
   return CYR_OK;
 }

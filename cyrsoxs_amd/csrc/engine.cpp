@@ -38,7 +38,10 @@ namespace {
 // Relative cost of one simulation, for scheduling: HBM bytes the executor moves.
 double sim_cost(const cyr_sim_desc *d, const SimGeometry &g) {
   const double V = (double)g.V, M = g.M, units = (double)g.nE * g.nK;
-  if (d->pipeline == CYR_PIPE_EBASIS && ebasis_usable(d, g)) return units * V * (16.0 * M + 80.0);
+  const bool fused = d->pipeline != CYR_PIPE_PLAIN && ebasis_usable(d, g);
+  if (fused && d->pipeline == CYR_PIPE_MBASIS)  // basis passes per k, then 2D work per energy
+    return g.nK * V * (16.0 * M + 7.0 * M * 8.0 * 6.0) + units * g.P * (16.0 * 7.0 * M + 8.0 * g.nA);
+  if (fused) return units * V * (16.0 * M + 80.0);
   return units * g.nA * V * (16.0 * M + 24.0 + 96.0);
 }
 

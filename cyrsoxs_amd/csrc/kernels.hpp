@@ -76,9 +76,12 @@ uint32_t ebasis_max_chunks(uint32_t nx, uint32_t ny, uint32_t nz, int precision)
 // linear interpolation the upper bin is kzt - 1 (mod Nz).
 cyr_status launch_ebasis_bins(const EwaldGeom &g, int32_t *kzt, hipStream_t st);
 
+// Outputs of the fields + X FFT kernel: 5 tensor fields or up to 7 packed basis volumes.
+constexpr int kXMaxOut = 8;
 template <typename T> struct PolxArgs {
-  const MatConst<T> *mats;
+  const MatConst<T> *mats;         // tensor mode: this energy's M constants
   uint32_t M;
+  uint32_t m0;                     // basis modes: first material of the pass
   uint64_t V;
   uint32_t nx, ny, nz, rows;       // rows = ny * nz
   const T *hx, *hy, *hz;           // Hanning tables
@@ -86,7 +89,7 @@ template <typename T> struct PolxArgs {
   int32_t rot;                     // 0 when Q is the identity
   const cx<T> *twx;                // filled by the launcher
   uint32_t tpp, pp, pitch;         // filled by the launcher
-  cx<T> *out[kEbFields];           // X-transformed tensor fields
+  cx<T> *out[kXMaxOut];            // X-transformed fields
 };
 template <typename T>
 cyr_status launch_ebasis_polx(PolxArgs<T> a, const void *vox, int storage, int morph, bool hann, hipStream_t st);
@@ -116,10 +119,11 @@ template <typename T> cyr_status launch_ebasis_yz(YzArgs<T> a, int ng, hipStream
 
 // DC-replacement neighbour sums per (field, z) slice, FP64.
 template <typename T> struct DcArgs {
-  const cx<T> *vol[kEbFields];     // X-transformed fields
+  const cx<T> *vol[kXMaxOut];      // X-transformed fields
+  uint32_t count;                  // volumes (0 = kEbFields)
   const cx<T> *twy, *twz;          // twz null in 2D
   uint32_t nx, ny, nz;
-  double2 *dcz;                    // [kEbFields][Nz]
+  double2 *dcz;                    // [count][Nz]
 };
 template <typename T> cyr_status launch_ebasis_dc(const DcArgs<T> &a, hipStream_t st);
 
@@ -133,6 +137,45 @@ template <typename T> struct AnglesArgs {
   T *proj;                         // [nA][Ny][Nx]
 };
 template <typename T> cyr_status launch_ebasis_angles(const AnglesArgs<T> &a, int ng, hipStream_t st);
+
+// ---- material-basis executor (mbasis.hip) ----------------------------------
+// Packed basis volumes per k: 7 per material pair, 4 for an odd last material.
+inline uint32_t mbasis_volumes(uint32_t M) { return 7 * (M / 2) + 4 * (M % 2); }
+// Per column (FFT-index order): Zlo | (width << 16) of the shifted z indices the
+// energies [0, ne) sample (kmag[e] = |k|); bmax[0] = widest band, bmax[1..] scratch
+// of mbasis_bmax_words - 1 entries.
+cyr_status launch_mbasis_bands(const EwaldGeom &g, const double *kmag, uint32_t ne, int32_t *band, uint32_t *bmax,
+                               hipStream_t st);
+uint64_t mbasis_bmax_words(uint32_t nx, uint32_t ny);
+// Fields + X FFT of one material pass: a pair (m0, m0 + 1, 7 outputs) or a single
+// material (m0, 4 outputs).
+template <typename T>
+cyr_status launch_mbasis_xfields(PolxArgs<T> a, bool pair, const void *vox, int storage, int morph, bool hann,
+                                 hipStream_t st);
+template <typename T> struct BandArgs {
+  const cx<T> *vol[kXMaxOut];      // transformed packed volumes of this pass
+  uint32_t count, vbase;           // volumes, global index of vol[0]
+  const int32_t *band;             // [Ny][Nx] band table
+  cx<T> *own, *mir;                // [volumes][J][Ny][Nx]
+  uint32_t J, nx, ny, nz;
+};
+template <typename T> cyr_status launch_mbasis_extract(const BandArgs<T> &a, hipStream_t st);
+// DC-replacement values of the packed volumes from their per-slice neighbour sums.
+cyr_status launch_mbasis_dcsum(const double2 *dcz, uint32_t nvol, uint32_t nz, bool three, double2 *dcv,
+                               hipStream_t st);
+template <typename T> struct MbAnglesArgs {
+  EwaldGeom g;                     // this energy's geometry
+  const int32_t *band;
+  const cx<T> *own, *mir;
+  uint32_t J;
+  const double2 *dcv;              // [volumes]
+  const MatConst<T> *mats;         // this energy's M constants
+  uint32_t M;
+  const T *amat;                   // [nA][kEbAmat]
+  uint32_t nA;
+  T *proj;                         // [nA][Ny][Nx]
+};
+template <typename T> cyr_status launch_mbasis_angles(const MbAnglesArgs<T> &a, int ng, hipStream_t st);
 
 // ---- rotation / averaging (rotate.hip) --------------------------------------
 // Plain path: rotate one projection by the E angle and accumulate (FP64).

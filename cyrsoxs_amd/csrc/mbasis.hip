@@ -1,0 +1,323 @@
+// Kernels of the material-basis executor.
+//
+// The E-basis tensor fields of an energy are linear in that energy's optical
+// constants with energy-independent real coefficients (xfields.hpp):
+//   N'00 = sum_m b_m alpha_m + c_m beta_m + d_m q00_m,   N'01 = sum_m d_m q01_m, ...
+// Windowing, the 3D FFT and the DC replacement are linear, so per k vector the
+// seven real basis fields of every material are transformed once and each
+// energy's transformed fields are formed at its Ewald bins as
+//   F_c(k) = sum over basis fields of (constant of the energy) x (basis spectrum)(k).
+// Two real fields A, B share one complex volume Z = A + iB; with
+// Zm(k) = Z(-k) their spectra are A^(k) = (Z + conj Zm) / 2 and
+// B^(k) = (Z - conj Zm) / 2i, so every column keeps Z at its own Ewald bins
+// and at the mirrored bins of the opposite column.
+//
+// Launches per (k, energy group): mbasis_bands_kernel (+ band width reduction),
+// then per material pass xfields_kernel (pair/single mode) -> DC neighbour sums
+// -> Y and Z FFT passes -> mbasis_extract_kernel; mbasis_dcsum_kernel; then per
+// energy mbasis_angles_kernel and the shared rotation / detector kernels.
+#include <hip/hip_runtime.h>
+
+#include "geom.hpp"
+#include "kernels.hpp"
+#include "xfields.hpp"
+
+namespace cyr {
+
+namespace {
+
+template <typename T> __device__ inline T mb_qnan();
+template <> __device__ inline float mb_qnan<float>() { return __int_as_float(0x7fc00000); }
+template <> __device__ inline double mb_qnan<double>() { return __longlong_as_double(0x7ff8000000000000ll); }
+
+// ---- band of shifted z indices per column over the energy group ---------------
+// band[col] = Zlo | (width << 16) over every energy's Ewald z index (and its
+// linear-interpolation partner z + 1); 0 = no energy samples the column.  Each
+// block also writes the widest band it saw (no atomics: a second kernel reduces).
+__global__ void __launch_bounds__(256) mbasis_bands_kernel(EwaldGeom g, const double *__restrict__ kmag, uint32_t ne,
+                                                           int32_t *__restrict__ band, uint32_t *__restrict__ bmax) {
+  __shared__ uint32_t red[256];
+  const uint64_t P = (uint64_t)g.nx * g.ny;
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t w = 0;
+  if (t < P) {
+    const uint32_t Y = (uint32_t)(t / g.nx), X = (uint32_t)(t - (uint64_t)Y * g.nx);
+    uint32_t lo = 0xffffffffu, hi = 0;
+    for (uint32_t e = 0; e < ne; e++) {
+      EwaldGeom ge = g;
+      ge.k = kmag[e];
+      uint32_t z0 = 0;
+      double f = 0;
+      const int kind = ewald_geometry(ge, X, Y, &z0, &f);
+      if (kind == 0) continue;
+      lo = z0 < lo ? z0 : lo;
+      const uint32_t top = kind == 2 ? z0 + 1 : z0;
+      hi = top > hi ? top : hi;
+    }
+    w = lo == 0xffffffffu ? 0 : hi - lo + 1;
+    const uint64_t col = (uint64_t)unshift(Y, g.ny) * g.nx + unshift(X, g.nx);
+    band[col] = w ? (int32_t)(lo | (w << 16)) : 0;
+  }
+  red[threadIdx.x] = w;
+  __syncthreads();
+  for (uint32_t s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] = red[threadIdx.x + s] > red[threadIdx.x] ? red[threadIdx.x + s] : red[threadIdx.x];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) bmax[1 + blockIdx.x] = red[0];
+}
+
+// bmax[0] = max over the n per-block entries bmax[1..n]
+__global__ void __launch_bounds__(256) mbasis_bmax_kernel(uint32_t *__restrict__ bmax, uint32_t n) {
+  __shared__ uint32_t red[256];
+  uint32_t m = 0;
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) m = bmax[1 + i] > m ? bmax[1 + i] : m;
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (uint32_t s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] = red[threadIdx.x + s] > red[threadIdx.x] ? red[threadIdx.x + s] : red[threadIdx.x];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) bmax[0] = red[0];
+}
+
+// ---- band values of the transformed packed volumes ----------------------------
+// own[v][j][col] = Z_v(col, kz(Zlo + j)),  mir[v][j][col] = Z_v(-col, -kz(Zlo + j))
+// for j < width(col).  Grid: (columns, j, volume).
+template <typename T>
+__global__ void __launch_bounds__(256) mbasis_extract_kernel(BandArgs<T> a) {
+  const uint64_t P = (uint64_t)a.nx * a.ny;
+  const uint64_t col = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t j = blockIdx.y, v = blockIdx.z;
+  if (col >= P) return;
+  const uint32_t b = (uint32_t)a.band[col];
+  if (j >= (b >> 16)) return;
+  const uint32_t kz = a.nz > 1 ? unshift((b & 0xffffu) + j, a.nz) : 0;
+  const uint32_t iy = (uint32_t)(col / a.nx), ix = (uint32_t)(col - (uint64_t)iy * a.nx);
+  const uint32_t mx = ix ? a.nx - ix : 0, my = iy ? a.ny - iy : 0, mz = kz ? a.nz - kz : 0;
+  const cx<T> *Z = a.vol[v];
+  const uint64_t o = ((uint64_t)(a.vbase + v) * a.J + j) * P + col;
+  a.own[o] = Z[(uint64_t)kz * P + col];
+  a.mir[o] = Z[(uint64_t)mz * P + (uint64_t)my * a.nx + mx];
+}
+
+// dcv[v] = (sum_z dcz[v][z]) / cnt, fixed-order tree (deterministic); one block per volume
+__global__ void __launch_bounds__(256) mbasis_dcsum_kernel(const double2 *__restrict__ dcz, uint32_t nz, double cnt,
+                                                           double2 *__restrict__ dcv) {
+  __shared__ double red[2][256];
+  const uint32_t v = blockIdx.x;
+  double sr = 0, si = 0;
+  for (uint32_t z = threadIdx.x; z < nz; z += blockDim.x) {
+    const double2 x = dcz[(uint64_t)v * nz + z];
+    sr += x.x;
+    si += x.y;
+  }
+  red[0][threadIdx.x] = sr;
+  red[1][threadIdx.x] = si;
+  __syncthreads();
+  for (uint32_t s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      red[0][threadIdx.x] += red[0][threadIdx.x + s];
+      red[1][threadIdx.x] += red[1][threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double2 o;
+    o.x = red[0][0] / cnt;
+    o.y = red[1][0] / cnt;
+    dcv[v] = o;
+  }
+}
+
+template <typename T> __device__ inline void cmac(cx<T> &acc, T cr, T ci, const cx<T> &x) {
+  acc.x += cr * x.x - ci * x.y;
+  acc.y += cr * x.y + ci * x.x;
+}
+
+// Spectra of the two real fields packed in one volume, from Z(k) and Z(-k).
+template <typename T> __device__ inline void unpack(const cx<T> &z, const cx<T> &zm, cx<T> &a, cx<T> &b) {
+  const T h = (T)0.5;
+  a = mk<T>(h * (z.x + zm.x), h * (z.y - zm.y));
+  b = mk<T>(h * (z.y + zm.y), -h * (z.x - zm.x));
+}
+
+// Adds basis field r (alpha, beta, q00, q11, q01, q02, q12) of material constant mc.
+template <typename T>
+__device__ inline void add_basis(int r, const MatConst<T> &mc, const cx<T> &x, cx<T> &iso, cx<T> F[kEbFields]) {
+  switch (r) {
+    case 0: cmac<T>(iso, mc.br, mc.bi, x); break;
+    case 1: cmac<T>(iso, mc.cr, mc.ci, x); break;
+    case 2: cmac<T>(F[0], mc.dr, mc.di, x); break;
+    case 3: cmac<T>(F[3], mc.dr, mc.di, x); break;
+    case 4: cmac<T>(F[1], mc.dr, mc.di, x); break;
+    case 5: cmac<T>(F[2], mc.dr, mc.di, x); break;
+    default: cmac<T>(F[4], mc.dr, mc.di, x); break;
+  }
+}
+
+// ---- per energy: fields at the Ewald bins, every angle's intensity ---------------
+template <typename T, int NG>
+__global__ void __launch_bounds__(256) mbasis_angles_kernel(MbAnglesArgs<T> a) {
+  const EwaldGeom &g = a.g;
+  const uint64_t P2 = (uint64_t)g.nx * g.ny;
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= P2) return;
+  const uint32_t Y = (uint32_t)(t / g.nx), X = (uint32_t)(t - (uint64_t)Y * g.nx);
+  uint32_t z0 = 0;
+  double f = 0;
+  const int kind = ewald_geometry(g, X, Y, &z0, &f);
+  if (kind == 0) {
+    for (uint32_t an = 0; an < a.nA; an++) a.proj[(uint64_t)an * P2 + t] = mb_qnan<T>();
+    return;
+  }
+  const uint32_t ix = unshift(X, g.nx), iy = unshift(Y, g.ny);
+  const uint64_t col = (uint64_t)iy * g.nx + ix;
+  const bool three = !g.enable2d;
+  const uint32_t j0 = three ? z0 - ((uint32_t)a.band[col] & 0xffffu) : 0;
+  // the DC-replaced value stands in for Z(0) (its own mirror) at FFT bin (0, 0, 0)
+  bool dc[NG];
+#pragma unroll
+  for (int q = 0; q < NG; q++) dc[q] = ix == 0 && iy == 0 && (three ? unshift(z0 + q, g.nz) == 0 : true);
+  const uint64_t slab = (uint64_t)a.J * P2;
+  const cx<T> *own = a.own + (uint64_t)j0 * P2 + col, *mir = a.mir + (uint64_t)j0 * P2 + col;
+  auto fetch = [&](uint32_t v, int q, cx<T> &z, cx<T> &zm) {
+    if (dc[q]) {
+      const double2 d = a.dcv[v];
+      z = zm = mk<T>((T)d.x, (T)d.y);
+    } else {
+      z = own[(uint64_t)v * slab + (uint64_t)q * P2];
+      zm = mir[(uint64_t)v * slab + (uint64_t)q * P2];
+    }
+  };
+  cx<T> F[NG][kEbFields], iso[NG];
+#pragma unroll
+  for (int q = 0; q < NG; q++) {
+    iso[q] = mk<T>(0, 0);
+#pragma unroll
+    for (int c = 0; c < kEbFields; c++) F[q][c] = mk<T>(0, 0);
+  }
+  const uint32_t pairs = a.M / 2;
+  for (uint32_t p = 0; p < pairs; p++) {
+    const MatConst<T> ma = a.mats[2 * p], mb = a.mats[2 * p + 1];
+#pragma unroll
+    for (int r = 0; r < 7; r++) {
+#pragma unroll
+      for (int q = 0; q < NG; q++) {
+        cx<T> z, zm, xa, xb;
+        fetch(7 * p + r, q, z, zm);
+        unpack<T>(z, zm, xa, xb);
+        add_basis<T>(r, ma, xa, iso[q], F[q]);
+        add_basis<T>(r, mb, xb, iso[q], F[q]);
+      }
+    }
+  }
+  if (a.M & 1) {  // last material alone: (alpha, beta), (q00, q11), (q01, q02), (q12, -)
+    const MatConst<T> mc = a.mats[a.M - 1];
+    const uint32_t vb = 7 * pairs;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+#pragma unroll
+      for (int q = 0; q < NG; q++) {
+        cx<T> z, zm, xa, xb;
+        fetch(vb + r, q, z, zm);
+        unpack<T>(z, zm, xa, xb);
+        add_basis<T>(2 * r, mc, xa, iso[q], F[q]);
+        if (r < 3) add_basis<T>(2 * r + 1, mc, xb, iso[q], F[q]);
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < NG; q++) {
+    F[q][0].x += iso[q].x;
+    F[q][0].y += iso[q].y;
+    F[q][3].x += iso[q].x;
+    F[q][3].y += iso[q].y;
+  }
+  double md[NG][6];
+#pragma unroll
+  for (int q = 0; q < NG; q++) scatter_matrix(g, X, Y, three ? z0 + q : 0, md[q]);
+  const T w1 = (T)f;
+#pragma unroll 4
+  for (uint32_t an = 0; an < a.nA; an++) {
+    const T *A = a.amat + (uint64_t)kEbAmat * an;
+    T I[NG];
+#pragma unroll
+    for (int q = 0; q < NG; q++) {
+      cx<T> pv[3];
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+        T re = 0, im = 0;
+#pragma unroll
+        for (int j = 0; j < kEbFields; j++) {
+          re += A[i * kEbFields + j] * F[q][j].x;
+          im += A[i * kEbFields + j] * F[q][j].y;
+        }
+        pv[i] = mk<T>(re, im);
+      }
+      I[q] = scatter_intensity<T>(md[q], pv);
+    }
+    T out;
+    if constexpr (NG == 2) out = (1 - w1) * I[0] + w1 * I[1];
+    else out = I[0];
+    a.proj[(uint64_t)an * P2 + t] = out;
+  }
+}
+
+}  // namespace
+
+cyr_status launch_mbasis_bands(const EwaldGeom &g, const double *kmag, uint32_t ne, int32_t *band, uint32_t *bmax,
+                               hipStream_t st) {
+  const uint64_t P = (uint64_t)g.nx * g.ny;
+  const uint32_t blocks = ceil_div(P, 256);
+  hipLaunchKernelGGL(mbasis_bands_kernel, dim3(blocks), dim3(256), 0, st, g, kmag, ne, band, bmax);
+  CYR_LAUNCHED();
+  hipLaunchKernelGGL(mbasis_bmax_kernel, dim3(1), dim3(256), 0, st, bmax, blocks);
+  CYR_LAUNCHED();
+  return CYR_OK;
+}
+
+uint64_t mbasis_bmax_words(uint32_t nx, uint32_t ny) { return 1 + ceil_div((uint64_t)nx * ny, 256); }
+
+template <typename T> cyr_status launch_mbasis_extract(const BandArgs<T> &a, hipStream_t st) {
+  if (a.count == 0) return CYR_OK;
+  if (a.count > (uint32_t)kXMaxOut || a.J == 0 || a.J > 65535) return fail(CYR_E_INVALID, "band extraction: bad shape");
+  const uint64_t P = (uint64_t)a.nx * a.ny;
+  hipLaunchKernelGGL((mbasis_extract_kernel<T>), dim3(ceil_div(P, 256), a.J, a.count), dim3(256), 0, st, a);
+  CYR_LAUNCHED();
+  return CYR_OK;
+}
+
+cyr_status launch_mbasis_dcsum(const double2 *dcz, uint32_t nvol, uint32_t nz, bool three, double2 *dcv,
+                               hipStream_t st) {
+  if (nvol == 0) return CYR_OK;
+  hipLaunchKernelGGL(mbasis_dcsum_kernel, dim3(nvol), dim3(256), 0, st, dcz, nz, three ? 6.0 : 4.0, dcv);
+  CYR_LAUNCHED();
+  return CYR_OK;
+}
+
+template <typename T> cyr_status launch_mbasis_angles(const MbAnglesArgs<T> &a, int ng, hipStream_t st) {
+  const uint64_t P = (uint64_t)a.g.nx * a.g.ny;
+  const dim3 grid(ceil_div(P, 256)), block(256);
+  if (ng == 2) hipLaunchKernelGGL((mbasis_angles_kernel<T, 2>), grid, block, 0, st, a);
+  else hipLaunchKernelGGL((mbasis_angles_kernel<T, 1>), grid, block, 0, st, a);
+  CYR_LAUNCHED();
+  return CYR_OK;
+}
+
+template <typename T>
+cyr_status launch_mbasis_xfields(PolxArgs<T> a, bool pair, const void *vox, int storage, int morph, bool hann,
+                                 hipStream_t st) {
+  if (pair) return launch_xfields<T, kXPair>(a, vox, storage, morph, hann, st);
+  return launch_xfields<T, kXSingle>(a, vox, storage, morph, hann, st);
+}
+
+#define CYR_INST(T)                                                                                          \
+  template cyr_status launch_mbasis_extract<T>(const BandArgs<T> &, hipStream_t);                          \
+  template cyr_status launch_mbasis_angles<T>(const MbAnglesArgs<T> &, int, hipStream_t);                  \
+  template cyr_status launch_mbasis_xfields<T>(PolxArgs<T>, bool, const void *, int, int, bool, hipStream_t);
+CYR_INST(float)
+CYR_INST(double)
+#undef CYR_INST
+
+}  // namespace cyr

@@ -78,7 +78,7 @@ template <typename T> MatConst<T> mat_const(const double *m4) {
 
 // Per-simulation constant tables, uploaded once.
 struct Tables {
-  size_t mats = 0, han = 0, amat = 0, rotinv = 0, detinv = 0, total = 0;
+  size_t mats = 0, han = 0, amat = 0, rotinv = 0, detinv = 0, kmag = 0, total = 0;
   std::vector<unsigned char> host;
 };
 
@@ -139,6 +139,8 @@ template <typename T> void build_layout(const SimGeometry &g, Tables &tb) {
   off = align256(off + sizeof(double) * 6 * g.nK * g.nA);
   tb.detinv = off;
   off = align256(off + sizeof(double) * 6 * g.nK);
+  tb.kmag = off;
+  off = align256(off + sizeof(double) * g.nE);
   tb.total = off;
 }
 
@@ -164,6 +166,7 @@ void build_tables(const cyr_sim_desc *d, const SimGeometry &g, const double *mat
     }
   std::memcpy(tb.host.data() + tb.rotinv, g.rot_inv.data(), sizeof(double) * 6 * g.nK * g.nA);
   std::memcpy(tb.host.data() + tb.detinv, g.det_inv.data(), sizeof(double) * 6 * g.nK);
+  std::memcpy(tb.host.data() + tb.kmag, g.kmag.data(), sizeof(double) * g.nE);
 }
 
 template <typename T> Vec3<T> v3(const V3 &v) { return Vec3<T>{(T)v.x, (T)v.y, (T)v.z}; }
@@ -177,7 +180,7 @@ template <typename T> Mat3<T> m3(const M3 &m) {
 template <typename T> struct Setup {
   const MatConst<T> *mats;
   const T *hx, *hy, *hz, *amat;
-  const double *rotinv;
+  const double *rotinv, *kmag;
   const cx<T> *twy, *twz;
   double vox_bytes, cxb;
 };
@@ -196,6 +199,7 @@ cyr_status setup_views(const cyr_sim_desc *d, const SimGeometry &g, const Tables
   su.hz = han + g.nx + g.ny;
   su.amat = reinterpret_cast<const T *>(tbase + tb.amat);
   su.rotinv = reinterpret_cast<const double *>(tbase + tb.rotinv);
+  su.kmag = reinterpret_cast<const double *>(tbase + tb.kmag);
   cyr_status s;
   su.twz = g.nz > 1 ? twiddles<T>(g.nz, &s) : nullptr;
   if (g.nz > 1 && !su.twz) return s;
@@ -368,7 +372,7 @@ cyr_status run_ebasis(const cyr_sim_desc *d, const SimGeometry &g, const void *d
     for (int c = 0; c < kEbFields; c++) xa.out[c] = pol + (uint64_t)c * V;
     kt_begin(s);
     CYR_TRY(launch_ebasis_polx<T>(xa, d_vox, d->storage, d->morphology_type, hann, s));
-    kt_end("ebasis_polx", s, su.vox_bytes + kEbFields * (double)V * cxb);
+    kt_end("xfields", s, su.vox_bytes + kEbFields * (double)V * cxb);
     if (lanes > 1) CYR_HIP(hipEventRecord(wb.ev_polx[L], s));
     DcArgs<T> da{};
     for (int c = 0; c < kEbFields; c++) da.vol[c] = pol + (uint64_t)c * V;
@@ -412,6 +416,189 @@ cyr_status run_ebasis(const cyr_sim_desc *d, const SimGeometry &g, const void *d
   return CYR_OK;
 }
 
+// Widest Ewald band over energies [e0, e0 + ne) for k, by (grid, k, |k| list):
+// known after one eager run, so graph capture can size its launches.
+std::mutex g_band_mu;
+std::map<std::vector<double>, uint32_t> g_band_widths;
+
+// Band table of energies [e0, e0 + ne) for k vector k into `band`, and its widest
+// band in *J (read back once per distinct geometry; resident runs need it cached).
+cyr_status mbasis_band(const cyr_sim_desc *d, const SimGeometry &g, uint32_t k, uint32_t e0, uint32_t ne,
+                       const double *d_kmag, int32_t *band, uint32_t *bmax, hipStream_t s, RunMode mode,
+                       uint32_t *J) {
+  const double kv[3] = {g.kvec[k].x, g.kvec[k].y, g.kvec[k].z};
+  const EwaldGeom geo = make_ewald_geom(g.nx, g.ny, g.nz, d->phys_size, g.kmag[e0], kv, d->interpolation);
+  CYR_TRY(launch_mbasis_bands(geo, d_kmag + e0, ne, band, bmax, s));
+  std::vector<double> key = {(double)g.nx, (double)g.ny, (double)g.nz, d->phys_size, (double)d->interpolation,
+                             kv[0], kv[1], kv[2]};
+  key.insert(key.end(), g.kmag.begin() + e0, g.kmag.begin() + e0 + ne);
+  {
+    std::lock_guard<std::mutex> lk(g_band_mu);
+    auto it = g_band_widths.find(key);
+    if (it != g_band_widths.end()) {
+      *J = it->second;
+      return CYR_OK;
+    }
+  }
+  if (mode == kRunResident)
+    return fail(CYR_E_INVALID, "material basis: Ewald band widths unknown under capture (run the simulation once first)");
+  uint32_t h = 0;
+  CYR_HIP(hipMemcpyAsync(&h, bmax, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  CYR_HIP(hipStreamSynchronize(s));
+  std::lock_guard<std::mutex> lk(g_band_mu);
+  g_band_widths[key] = h;
+  *J = h;
+  return CYR_OK;
+}
+
+// Largest band kept per energy group: the band values then take no more memory
+// than the seven transformed volumes of a material pass.
+uint32_t mbasis_band_cap(const SimGeometry &g) {
+  const uint32_t nv = mbasis_volumes(g.M);
+  const uint32_t cap = (uint32_t)((7ull * g.nz) / (2ull * nv));
+  const uint32_t floor_cap = g.nz < 8 ? g.nz : 8;
+  return cap > floor_cap ? cap : floor_cap;
+}
+
+// MBASIS: per (k, energy group) the seven real basis fields of every material
+// (packed two per complex volume), their 3D FFTs, and their values in the band
+// of Ewald bins the group samples (plus the mirrored bins); then per energy the
+// transformed tensor fields at its bins as combinations with its constants, and
+// every angle as 2D work.
+template <typename T>
+cyr_status run_mbasis(const cyr_sim_desc *d, const SimGeometry &g, const void *d_vox, T *d_out, WorkBuffers &wb,
+                      hipStream_t s, const Setup<T> &su, RunMode mode) {
+  const uint64_t V = g.V, P = g.P;
+  const bool hann = d->windowing == 1 && d->algorithm != 1;
+  const bool three = g.nz > 1;
+  const int ng = (three && d->interpolation == 1) ? 2 : 1;
+  const uint32_t nvol = mbasis_volumes(g.M), pass_max = nvol < 7 ? nvol : 7;
+  LaneBuffers &lb = wb.lane[0];
+  CYR_TRY(ensure_buffer(lb.pol, sizeof(cx<T>) * V * pass_max));
+  CYR_TRY(ensure_buffer(lb.proj, sizeof(T) * P * g.nA));
+  CYR_TRY(ensure_buffer(lb.avg, sizeof(double) * P));
+  CYR_TRY(ensure_buffer(lb.bins, sizeof(int32_t) * P + sizeof(uint32_t) * mbasis_bmax_words(g.nx, g.ny)));
+  CYR_TRY(ensure_buffer(lb.dc, sizeof(double2) * nvol * (g.nz + 1)));
+  cx<T> *pol = reinterpret_cast<cx<T> *>(lb.pol.ptr);
+  int32_t *band = reinterpret_cast<int32_t *>(lb.bins.ptr);
+  uint32_t *bmax = reinterpret_cast<uint32_t *>(band + P);
+  double2 *dcz = reinterpret_cast<double2 *>(lb.dc.ptr);
+  double2 *dcv = dcz + (uint64_t)nvol * g.nz;
+  T *proj = reinterpret_cast<T *>(lb.proj.ptr);
+  double *avg = reinterpret_cast<double *>(lb.avg.ptr);
+  const double cxb = su.cxb;
+  const uint32_t cap = mbasis_band_cap(g);
+  const double vel = d->storage == CYR_STORE_NATIVE ? (double)sizeof(T) : 2.0;
+
+  for (uint32_t k = 0; k < g.nK; k++) {
+    const M3 Q = ebasis_frame(g, k);
+    for (uint32_t e0 = 0; e0 < g.nE;) {
+      uint32_t ne = g.nE - e0, J = 0;
+      for (;;) {
+        kt_begin(s);
+        CYR_TRY(mbasis_band(d, g, k, e0, ne, su.kmag, band, bmax, s, mode, &J));
+        kt_end("mbasis_bands", s, (double)P * 4);
+        if (J <= cap || ne == 1) break;
+        ne = (ne + 1) / 2;
+      }
+      if (J == 0) J = 1;  // no pixel sampled: the angles kernel writes NaN only
+      CYR_TRY(ensure_buffer(lb.part, sizeof(cx<T>) * 2 * (uint64_t)nvol * J * P));
+      cx<T> *own = reinterpret_cast<cx<T> *>(lb.part.ptr);
+      cx<T> *mir = own + (uint64_t)nvol * J * P;
+      // ---- basis spectra in the band: one pass per material pair (or single)
+      for (uint32_t m0 = 0, vb = 0; m0 < g.M;) {
+        const bool pair = m0 + 1 < g.M;
+        const uint32_t nv = pair ? 7 : 4;
+        PolxArgs<T> xa{};
+        xa.M = g.M;
+        xa.m0 = m0;
+        xa.V = V;
+        xa.nx = g.nx;
+        xa.ny = g.ny;
+        xa.nz = g.nz;
+        xa.rows = g.ny * g.nz;
+        xa.hx = su.hx;
+        xa.hy = su.hy;
+        xa.hz = su.hz;
+        xa.Q = m3<T>(Q);
+        xa.rot = is_identity(Q) ? 0 : 1;
+        void *vptr[kXMaxOut];
+        for (uint32_t c = 0; c < nv; c++) {
+          xa.out[c] = pol + (uint64_t)c * V;
+          vptr[c] = xa.out[c];
+        }
+        kt_begin(s);
+        CYR_TRY(launch_mbasis_xfields<T>(xa, pair, d_vox, d->storage, d->morphology_type, hann, s));
+        kt_end("xfields", s, (pair ? 2.0 : 1.0) * 4.0 * vel * V + nv * (double)V * cxb);
+        DcArgs<T> da{};
+        for (uint32_t c = 0; c < nv; c++) da.vol[c] = xa.out[c];
+        da.count = nv;
+        da.twy = su.twy;
+        da.twz = su.twz;
+        da.nx = g.nx;
+        da.ny = g.ny;
+        da.nz = g.nz;
+        da.dcz = dcz + (uint64_t)vb * g.nz;
+        kt_begin(s);
+        CYR_TRY(launch_ebasis_dc<T>(da, s));
+        kt_end("ebasis_dc", s, 3.0 * nv * (double)g.ny * g.nz * cxb);
+        kt_begin(s);
+        CYR_TRY(fft_axis<T>(vptr, nv, g.nx, g.ny, g.nz, 1, s));
+        kt_end("fft_y", s, 2.0 * nv * (double)V * cxb);
+        if (three) {
+          kt_begin(s);
+          CYR_TRY(fft_axis<T>(vptr, nv, g.nx, g.ny, g.nz, 2, s));
+          kt_end("fft_z", s, 2.0 * nv * (double)V * cxb);
+        }
+        BandArgs<T> ba{};
+        for (uint32_t c = 0; c < nv; c++) ba.vol[c] = xa.out[c];
+        ba.count = nv;
+        ba.vbase = vb;
+        ba.band = band;
+        ba.own = own;
+        ba.mir = mir;
+        ba.J = J;
+        ba.nx = g.nx;
+        ba.ny = g.ny;
+        ba.nz = g.nz;
+        kt_begin(s);
+        CYR_TRY(launch_mbasis_extract<T>(ba, s));
+        kt_end("mbasis_extract", s);
+        m0 += pair ? 2 : 1;
+        vb += nv;
+      }
+      CYR_TRY(launch_mbasis_dcsum(dcz, nvol, g.nz, three, dcv, s));
+      // ---- per energy: fields at the Ewald bins, angles, rotation
+      for (uint32_t e = e0; e < e0 + ne; e++) {
+        const double kv[3] = {g.kvec[k].x, g.kvec[k].y, g.kvec[k].z};
+        const EwaldGeom geo = make_ewald_geom(g.nx, g.ny, g.nz, d->phys_size, g.kmag[e], kv, d->interpolation);
+        MbAnglesArgs<T> aa{};
+        aa.g = geo;
+        aa.band = band;
+        aa.own = own;
+        aa.mir = mir;
+        aa.J = J;
+        aa.dcv = dcv;
+        aa.mats = su.mats + (size_t)e * g.M;
+        aa.M = g.M;
+        aa.amat = su.amat + (size_t)kEbAmat * k * g.nA;
+        aa.nA = g.nA;
+        aa.proj = proj;
+        kt_begin(s);
+        CYR_TRY(launch_mbasis_angles<T>(aa, ng, s));
+        kt_end("mbasis_angles", s, 2.0 * ng * nvol * (double)P * cxb + (double)g.nA * P * sizeof(T));
+        kt_begin(s);
+        CYR_TRY(launch_rotate_all<T>(proj, su.rotinv + (size_t)6 * k * g.nA, g.nA, g.nx, g.ny, d->rot_mask, avg, s));
+        kt_end("rotate_all", s, (double)(g.nA + 1) * P * sizeof(T));
+        T *out = d_out + ((uint64_t)e * g.nK + k) * P;
+        CYR_TRY(launch_detector_warp<T>(avg, out, g.nx, g.ny, &g.det_inv[(size_t)k * 6], d->rot_mask, s));
+      }
+      e0 += ne;
+    }
+  }
+  return CYR_OK;
+}
+
 template <typename T>
 cyr_status run_impl(const cyr_sim_desc *d, const SimGeometry &g, const void *d_vox, const double *mats, T *d_out,
                     WorkBuffers &wb, hipStream_t st, RunMode mode, const void *resident_tables) {
@@ -427,8 +614,10 @@ cyr_status run_impl(const cyr_sim_desc *d, const SimGeometry &g, const void *d_v
   }
   Setup<T> su{};
   CYR_TRY(setup_views<T>(d, g, tb, tables, su));
-  const bool ebasis = d->pipeline == CYR_PIPE_EBASIS && ebasis_usable(d, g);
-  const cyr_status s = ebasis ? run_ebasis<T>(d, g, d_vox, d_out, wb, st, su) : run_plain<T>(d, g, d_vox, d_out, wb, st, su);
+  const bool fused = d->pipeline != CYR_PIPE_PLAIN && ebasis_usable(d, g);
+  const cyr_status s = !fused ? run_plain<T>(d, g, d_vox, d_out, wb, st, su)
+                       : d->pipeline == CYR_PIPE_MBASIS ? run_mbasis<T>(d, g, d_vox, d_out, wb, st, su, mode)
+                                                       : run_ebasis<T>(d, g, d_vox, d_out, wb, st, su);
   if (mode != kRunSync) return s;
   // drain the stream on errors too, so no queued launch outlives the call's buffers
   const hipError_t se = hipStreamSynchronize(st);
@@ -557,9 +746,16 @@ uint64_t output_bytes(const cyr_sim_desc *d, const SimGeometry &g) {
 
 uint64_t work_bytes(const cyr_sim_desc *d, const SimGeometry &g) {
   const uint64_t cs = d->precision == CYR_FP64 ? 16 : 8, rs = cs / 2;
-  const bool ebasis = d->pipeline == CYR_PIPE_EBASIS && ebasis_usable(d, g);
+  const bool fused = d->pipeline != CYR_PIPE_PLAIN && ebasis_usable(d, g);
   uint64_t b = 0;
-  if (ebasis) {  // per lane: fields, bin table, Y/z partial sums (chunks x 2 bins x fields), DC sums,
+  if (fused && d->pipeline == CYR_PIPE_MBASIS) {
+    // one pass of packed volumes, band values (own + mirror, at most the band cap),
+    // band table, DC sums, per-angle projections, average
+    const uint64_t nv = mbasis_volumes(g.M), pass = nv < 7 ? nv : 7;
+    const uint64_t J = mbasis_band_cap(g) > 2 ? mbasis_band_cap(g) : 2;
+    b += cs * g.V * pass + cs * 2 * nv * J * g.P + 4 * g.P + 4 * mbasis_bmax_words(g.nx, g.ny) +
+         16 * nv * (g.nz + 1) + rs * g.P * g.nA + 8 * g.P;
+  } else if (fused) {  // per lane: fields, bin table, Y/z partial sums (chunks x 2 bins x fields), DC sums,
                  // per-angle projections, average
     const uint64_t nch = ebasis_max_chunks(g.nx, g.ny, g.nz, d->precision);
     const uint64_t lanes = (uint64_t)g.nE * g.nK > 1 ? energy_lanes() : 1;

@@ -47,9 +47,14 @@ enum { CYR_STORE_NATIVE = 0, CYR_STORE_FP16 = 1, CYR_STORE_BF16 = 2 };
  * (polarization -> 3 full 3D FFTs -> DC/shift -> Ewald -> rotate/accumulate for
  * every E angle).  EBASIS uses the linearity of the pipeline in E: two
  * polarizations per energy, an XY FFT, and only the Ewald-sphere z bins of the
- * z transform; every angle is then a 2D operation.  Both produce the same
+ * z transform; every angle is then a 2D operation.  MBASIS also uses the
+ * linearity in the optical constants: per k vector it transforms seven
+ * energy-independent real fields per material once (packed two per complex
+ * volume), keeps their values in the band of Ewald bins all energies touch, and
+ * forms each energy's fields there as combinations with that energy's
+ * constants; the per-energy work is then 2D.  All three produce the same
  * patterns within floating-point rounding. */
-enum { CYR_PIPE_PLAIN = 0, CYR_PIPE_EBASIS = 1 };
+enum { CYR_PIPE_PLAIN = 0, CYR_PIPE_EBASIS = 1, CYR_PIPE_MBASIS = 2 };
 
 /* Mirrors InputData (include/Input/InputData.h:186-232) plus the compile-time
  * switches of the reference build.  Enumerations use the reference's values

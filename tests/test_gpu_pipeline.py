@@ -33,7 +33,10 @@ def _check(got, ref, prec):
     return err
 
 
-@pytest.mark.parametrize("pipeline", [capi.PIPE_PLAIN, capi.PIPE_EBASIS])
+PIPES = [capi.PIPE_PLAIN, capi.PIPE_EBASIS, capi.PIPE_MBASIS]
+
+
+@pytest.mark.parametrize("pipeline", PIPES)
 @pytest.mark.parametrize("prec", [capi.FP64, capi.FP32])
 def test_sphere_64(prec, pipeline):
     """The 64^3 two-material sphere, 3 energies x 6 angles, nearest neighbour."""
@@ -60,10 +63,12 @@ CASES = {
     "odd_grid": dict(dims=(30, 30, 10), M=2, morph=0, kw=dict(interpolation=1)),
     "radix3_384": dict(dims=(384, 384, 4), M=2, morph=0, kw=dict(interpolation=1, windowing=1)),
     "pow2_128": dict(dims=(128, 128, 6), M=2, morph=1, kw=dict(interpolation=0)),
+    "one_material": dict(dims=(32, 32, 16), M=1, morph=0, kw=dict(interpolation=1, windowing=1)),
+    "five_materials": dict(dims=(24, 24, 16), M=5, morph=1, kw=dict(interpolation=1, reference_frame=0)),
 }
 
 
-@pytest.mark.parametrize("pipeline", [capi.PIPE_PLAIN, capi.PIPE_EBASIS])
+@pytest.mark.parametrize("pipeline", PIPES)
 @pytest.mark.parametrize("prec", [capi.FP64, capi.FP32])
 @pytest.mark.parametrize("case", sorted(CASES))
 def test_modes(case, prec, pipeline):
@@ -95,7 +100,7 @@ def test_beam_divergence_kvectors(prec):
     kw = dict(interpolation=1, case_type=1)
     osim = O.Sim(dims, M, 5.0, [285.0], (0.0, 45.0, 135.0), kvectors=k, **kw)
     ref = O.simulate(osim, vox, mats)
-    for pipeline in (capi.PIPE_PLAIN, capi.PIPE_EBASIS):
+    for pipeline in PIPES:
         sim = capi.Sim(dims, M, 5.0, [285.0], (0.0, 45.0, 135.0), kvectors=k, precision=prec, pipeline=pipeline,
                        **kw)
         got = capi.simulate(sim, vox, mats)
@@ -117,7 +122,7 @@ def test_device_api_matches_host_api():
 def test_repeatable_bitwise():
     """No atomics in any accumulation: two runs are bit-identical."""
     c = synthetic.config0()
-    for pipeline in (capi.PIPE_PLAIN, capi.PIPE_EBASIS):
+    for pipeline in PIPES:
         sim = capi.Sim(c["dims"], 2, c["phys_size"], c["energies"], c["angles"], interpolation=0,
                        pipeline=pipeline)
         a = capi.simulate(sim, c["voxels"], c["mats"])
@@ -134,13 +139,51 @@ def test_half_precision_morphology_storage(storage, tol):
     mats = random_constants(2, M, 5)
     kw = dict(interpolation=1, windowing=1)
     ref = _oracle("half_storage", O.Sim(dims, M, 5.0, energies, (0, 30, 150), **kw), vox, mats)
-    for pipeline in (capi.PIPE_PLAIN, capi.PIPE_EBASIS):
+    for pipeline in PIPES:
         sim = capi.Sim(dims, M, 5.0, energies, (0, 30, 150), precision=capi.FP32, storage=storage, pipeline=pipeline,
                        **kw)
         got = capi.simulate(sim, vox, mats)
         err, same = pattern_rel_l2(got, ref)
         print(f"storage={storage} pipeline={pipeline}: rel L2 {err:.3e}")
         assert same and err < tol
+
+
+@pytest.mark.parametrize("prec", [capi.FP64, capi.FP32])
+def test_mbasis_energy_groups(prec):
+    """Energies spread far enough that their Ewald bands exceed the per-group cap:
+    the material-basis executor splits them into groups (one basis transform each)."""
+    dims, M = (32, 32, 32), 4
+    vox = random_euler(dims, M, 41)
+    energies = [200.0, 240.0, 290.0, 350.0, 420.0, 500.0]
+    mats = random_constants(len(energies), M, 6)
+    kw = dict(interpolation=1, windowing=1)
+    ref = _oracle("energy_groups", O.Sim(dims, M, 2.0, energies, (0.0, 40.0, 160.0), **kw), vox, mats)
+    sim = capi.Sim(dims, M, 2.0, energies, (0.0, 40.0, 160.0), precision=prec, pipeline=capi.PIPE_MBASIS, **kw)
+    capi.kernel_timing(True)
+    try:
+        got = capi.simulate(sim, vox, mats)
+        launches = {k: v["calls"] for k, v in capi.kernel_timing_report().items()}
+    finally:
+        capi.kernel_timing(False)
+    err = _check(got, ref, prec)
+    print(f"energy groups prec={prec}: rel L2 {err:.3e}, launches {launches}")
+    # more than one group: more band passes than the one a single group needs
+    assert launches.get("mbasis_bands", 0) > 1
+
+
+@pytest.mark.parametrize("prec", [capi.FP64, capi.FP32])
+def test_mbasis_matches_ebasis(prec):
+    """The two fused executors agree (same pipeline, linear steps in another order)."""
+    c = synthetic.config0()
+    kw = dict(interpolation=1, windowing=1)
+    out = {}
+    for pipeline in (capi.PIPE_EBASIS, capi.PIPE_MBASIS):
+        sim = capi.Sim(c["dims"], 2, c["phys_size"], c["energies"], c["angles"], precision=prec, pipeline=pipeline,
+                       **kw)
+        out[pipeline] = capi.simulate(sim, c["voxels"], c["mats"])
+    err, same = pattern_rel_l2(out[capi.PIPE_MBASIS], out[capi.PIPE_EBASIS])
+    assert same
+    assert err < (1e-11 if prec == capi.FP64 else 2e-5), err
 
 
 def test_energy_lanes_bitwise():
@@ -199,9 +242,10 @@ def test_thinfilm_precision(prec, storage, tol):
     mats = synthetic.lorentzian_constants(energies, 3, seed=3)
     angles = (0.0, 2.0, 358.0)
     ref = _oracle("thinfilm", O.Sim((n, n, 1), 3, 1.0, energies, angles, interpolation=1), vox, mats)
-    sim = capi.Sim((n, n, 1), 3, 1.0, energies, angles, interpolation=1, precision=prec, storage=storage,
-                   pipeline=capi.PIPE_EBASIS)
-    got = capi.simulate(sim, vox, mats)
-    err, same = pattern_rel_l2(got, ref)
-    print(f"thin film prec={prec} storage={storage}: rel L2 {err:.3e}")
-    assert same and err < tol
+    for pipeline in (capi.PIPE_EBASIS, capi.PIPE_MBASIS):
+        sim = capi.Sim((n, n, 1), 3, 1.0, energies, angles, interpolation=1, precision=prec, storage=storage,
+                       pipeline=pipeline)
+        got = capi.simulate(sim, vox, mats)
+        err, same = pattern_rel_l2(got, ref)
+        print(f"thin film prec={prec} storage={storage} pipeline={pipeline}: rel L2 {err:.3e}")
+        assert same and err < tol
