@@ -160,6 +160,10 @@ template <typename T> struct BandArgs {
   uint32_t J, nx, ny, nz;
 };
 template <typename T> cyr_status launch_mbasis_extract(const BandArgs<T> &a, hipStream_t st);
+// Z FFT of the Y-transformed volumes writing only the band values (vol[] is read,
+// not overwritten); *done = false when Nz has no compile-time plan (then run the
+// Z FFT pass and launch_mbasis_extract).
+template <typename T> cyr_status launch_mbasis_zband(const BandArgs<T> &a, hipStream_t st, bool *done);
 // DC-replacement values of the packed volumes from their per-slice neighbour sums.
 cyr_status launch_mbasis_dcsum(const double2 *dcz, uint32_t nvol, uint32_t nz, bool three, double2 *dcv,
                                hipStream_t st);

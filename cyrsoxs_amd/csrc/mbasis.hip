@@ -14,10 +14,14 @@
 //
 // Launches per (k, energy group): mbasis_bands_kernel (+ band width reduction),
 // then per material pass xfields_kernel (pair/single mode) -> DC neighbour sums
-// -> Y and Z FFT passes -> mbasis_extract_kernel; mbasis_dcsum_kernel; then per
-// energy mbasis_angles_kernel and the shared rotation / detector kernels.
+// -> Y FFT pass -> mbasis_zband_kernel (Z FFT writing only band values; for Nz
+// without a compile-time plan: Z FFT pass + mbasis_extract_kernel);
+// mbasis_dcsum_kernel; then per energy mbasis_angles_kernel and the shared
+// rotation / detector kernels.
 #include <hip/hip_runtime.h>
 
+#include "fft.hpp"
+#include "fftpolicy.hpp"
 #include "geom.hpp"
 #include "kernels.hpp"
 #include "xfields.hpp"
@@ -99,6 +103,66 @@ __global__ void __launch_bounds__(256) mbasis_extract_kernel(BandArgs<T> a) {
   const uint64_t o = ((uint64_t)(a.vbase + v) * a.J + j) * P + col;
   a.own[o] = Z[(uint64_t)kz * P + col];
   a.mir[o] = Z[(uint64_t)mz * P + (uint64_t)my * a.nx + mx];
+}
+
+// ---- Z FFT fused with the band extraction --------------------------------------
+// Same launch layout as the strided FFT pass (PP adjacent x columns of one y row
+// per block, z pencils, twiddles staged in LDS, XCD-ordered 1D grid), but the
+// transformed pencil is not written back: each output Z(col, kz) goes to own[]
+// when kz is in the column's band and to mir[] of the opposite column when -kz
+// is in that column's band.
+template <typename T, class F, int WG>
+__global__ void __launch_bounds__(WG) mbasis_zband_kernel(BandArgs<T> a, const cx<T> *__restrict__ tw) {
+  constexpr int E = F::kE, TPP = F::TPP, N = TPP * E, PP = WG / TPP;
+  extern __shared__ unsigned char smem_raw[];
+  cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
+  const uint32_t groups = (a.nx + PP - 1) / PP, per_vol = groups * a.ny;
+  const uint32_t q = xcd_tile(blockIdx.x, gridDim.x);
+  if (q >= per_vol * a.count) return;  // grid padding: the whole block leaves before any barrier
+  cx<T> *ltw = lds + (size_t)PP * N;
+  for (uint32_t i = threadIdx.x; i < (uint32_t)N; i += WG) ltw[i] = tw[i];
+  lds_sync();
+  const uint32_t p = threadIdx.x % PP, t = threadIdx.x / PP;
+  const uint32_t v = q / per_vol, tile = q % per_vol;
+  const uint32_t g = tile % groups, y = tile / groups;
+  const uint32_t x = g * PP + p;
+  const bool active = x < a.nx;
+  const uint64_t P = (uint64_t)a.nx * a.ny;
+  const uint64_t col = (uint64_t)y * a.nx + (active ? x : 0);
+  const cx<T> *base = a.vol[v] + col;
+  cx<T> r[E];
+#pragma unroll
+  for (int i = 0; i < E; i++) r[i] = base[(uint64_t)F::in_pos(t, i) * P];
+  F::run(r, t, p, lds, ltw);
+  if (!active) return;
+  const uint32_t b = (uint32_t)a.band[col];
+  const uint32_t lo = b & 0xffffu, w = b >> 16;
+  const uint32_t mx = x ? a.nx - x : 0, my = y ? a.ny - y : 0;
+  const uint64_t cm = (uint64_t)my * a.nx + mx;
+  const uint32_t bm = (uint32_t)a.band[cm];
+  const uint32_t lom = bm & 0xffffu, wm = bm >> 16;
+  cx<T> *own = a.own + (uint64_t)(a.vbase + v) * a.J * P;
+  cx<T> *mir = a.mir + (uint64_t)(a.vbase + v) * a.J * P;
+#pragma unroll
+  for (int i = 0; i < E; i++) {
+    const uint32_t kz = F::out_pos(t, i);
+    const uint32_t j = unshift(kz, N) - lo;  // wraps above w when below the band
+    if (j < w) own[(uint64_t)j * P + col] = r[i];
+    const uint32_t jm = unshift(kz ? N - kz : 0, N) - lom;
+    if (jm < wm) mir[(uint64_t)jm * P + cm] = r[i];
+  }
+}
+
+template <typename T, int N>
+hipError_t launch_zband_n(const BandArgs<T> &a, hipStream_t st, const cx<T> *tw) {
+  constexpr int PP = N <= 512 ? 16 : (N == 1024 ? 8 : 4);
+  using F = typename CtPlan<T, N, false, PP>::type;
+  constexpr int WG = PP * F::TPP;
+  const uint64_t tiles = (uint64_t)((a.nx + PP - 1) / PP) * a.ny * a.count;
+  if (tiles > 0x7ffffff0ull) return hipErrorInvalidConfiguration;
+  const size_t lds = ((size_t)PP * N + N) * sizeof(cx<T>);
+  hipLaunchKernelGGL((mbasis_zband_kernel<T, F, WG>), dim3(xcd_grid(tiles)), dim3(WG), lds, st, a, tw);
+  return hipGetLastError();
 }
 
 // dcv[v] = (sum_z dcz[v][z]) / cnt, fixed-order tree (deterministic); one block per volume
@@ -296,6 +360,34 @@ cyr_status launch_mbasis_dcsum(const double2 *dcz, uint32_t nvol, uint32_t nz, b
   return CYR_OK;
 }
 
+template <typename T> cyr_status launch_mbasis_zband(const BandArgs<T> &a, hipStream_t st, bool *done) {
+  *done = false;
+  const int prec = sizeof(T) == 8 ? CYR_FP64 : CYR_FP32;
+  if (a.count == 0 || a.nz < 2 || !ct_usable(a.nz, prec)) return CYR_OK;
+  if (a.count > (uint32_t)kXMaxOut || a.J == 0) return fail(CYR_E_INVALID, "Z band pass: bad shape");
+  const uint32_t pp = a.nz <= 512 ? 16 : (a.nz == 1024 ? 8 : 4);
+  if (((size_t)pp * a.nz + a.nz) * sizeof(cx<T>) > device_max_lds()) return CYR_OK;
+  cyr_status s;
+  const cx<T> *tw = twiddles<T>(a.nz, &s);
+  if (!tw) return s;
+  hipError_t e = hipSuccess;
+  switch (a.nz) {
+    case 64: e = launch_zband_n<T, 64>(a, st, tw); break;
+    case 128: e = launch_zband_n<T, 128>(a, st, tw); break;
+    case 256: e = launch_zband_n<T, 256>(a, st, tw); break;
+    case 384:
+      if constexpr (sizeof(T) == 4) e = launch_zband_n<T, 384>(a, st, tw);
+      break;
+    case 512: e = launch_zband_n<T, 512>(a, st, tw); break;
+    case 1024: e = launch_zband_n<T, 1024>(a, st, tw); break;
+    case 2048: e = launch_zband_n<T, 2048>(a, st, tw); break;
+    default: return CYR_OK;
+  }
+  if (e != hipSuccess) return hip_fail(e, "mbasis_zband_kernel", __FILE__, __LINE__);
+  *done = true;
+  return CYR_OK;
+}
+
 template <typename T> cyr_status launch_mbasis_angles(const MbAnglesArgs<T> &a, int ng, hipStream_t st) {
   const uint64_t P = (uint64_t)a.g.nx * a.g.ny;
   const dim3 grid(ceil_div(P, 256)), block(256);
@@ -314,6 +406,7 @@ cyr_status launch_mbasis_xfields(PolxArgs<T> a, bool pair, const void *vox, int 
 
 #define CYR_INST(T)                                                                                          \
   template cyr_status launch_mbasis_extract<T>(const BandArgs<T> &, hipStream_t);                          \
+  template cyr_status launch_mbasis_zband<T>(const BandArgs<T> &, hipStream_t, bool *);                    \
   template cyr_status launch_mbasis_angles<T>(const MbAnglesArgs<T> &, int, hipStream_t);                  \
   template cyr_status launch_mbasis_xfields<T>(PolxArgs<T>, bool, const void *, int, int, bool, hipStream_t);
 CYR_INST(float)

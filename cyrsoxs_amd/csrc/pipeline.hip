@@ -545,11 +545,6 @@ cyr_status run_mbasis(const cyr_sim_desc *d, const SimGeometry &g, const void *d
         kt_begin(s);
         CYR_TRY(fft_axis<T>(vptr, nv, g.nx, g.ny, g.nz, 1, s));
         kt_end("fft_y", s, 2.0 * nv * (double)V * cxb);
-        if (three) {
-          kt_begin(s);
-          CYR_TRY(fft_axis<T>(vptr, nv, g.nx, g.ny, g.nz, 2, s));
-          kt_end("fft_z", s, 2.0 * nv * (double)V * cxb);
-        }
         BandArgs<T> ba{};
         for (uint32_t c = 0; c < nv; c++) ba.vol[c] = xa.out[c];
         ba.count = nv;
@@ -561,9 +556,22 @@ cyr_status run_mbasis(const cyr_sim_desc *d, const SimGeometry &g, const void *d
         ba.nx = g.nx;
         ba.ny = g.ny;
         ba.nz = g.nz;
-        kt_begin(s);
-        CYR_TRY(launch_mbasis_extract<T>(ba, s));
-        kt_end("mbasis_extract", s);
+        bool fused_z = false;
+        if (three) {
+          kt_begin(s);
+          CYR_TRY(launch_mbasis_zband<T>(ba, s, &fused_z));
+          if (fused_z) kt_end("mbasis_zband", s, nv * (double)V * cxb);
+          if (!fused_z) {
+            kt_begin(s);
+            CYR_TRY(fft_axis<T>(vptr, nv, g.nx, g.ny, g.nz, 2, s));
+            kt_end("fft_z", s, 2.0 * nv * (double)V * cxb);
+          }
+        }
+        if (!fused_z) {
+          kt_begin(s);
+          CYR_TRY(launch_mbasis_extract<T>(ba, s));
+          kt_end("mbasis_extract", s);
+        }
         m0 += pair ? 2 : 1;
         vb += nv;
       }

@@ -65,6 +65,8 @@ CASES = {
     "pow2_128": dict(dims=(128, 128, 6), M=2, morph=1, kw=dict(interpolation=0)),
     "one_material": dict(dims=(32, 32, 16), M=1, morph=0, kw=dict(interpolation=1, windowing=1)),
     "five_materials": dict(dims=(24, 24, 16), M=5, morph=1, kw=dict(interpolation=1, reference_frame=0)),
+    "deep_z": dict(dims=(32, 24, 128), M=3, morph=0, kw=dict(interpolation=1, windowing=1)),
+    "deep_z_nn": dict(dims=(24, 32, 64), M=2, morph=1, kw=dict(interpolation=0)),
 }
 
 
