@@ -36,6 +36,7 @@ TRAFFIC = {}
 # timing labels of the library (cyr_kernel_timing) -> kernel symbol
 KERNEL_OF = {"xfields": "xfields_kernel", "fft_y": "fft_ct_kernel (Y pass)", "fft_z": "fft_ct_kernel (Z pass)",
              "mbasis_extract": "mbasis_extract_kernel", "mbasis_zband": "mbasis_zband_kernel", "mbasis_angles": "mbasis_angles_kernel",
+             "mbasis_quad": "mbasis_quad_kernel", "rotate_quad": "rotate_quad_kernel",
              "mbasis_bands": "mbasis_bands_kernel", "ebasis_yz": "ebasis_yz_kernel", "ebasis_dc": "ebasis_dc_kernel",
              "ebasis_angles": "ebasis_angles_kernel", "ebasis_bins": "ebasis_bins_kernel",
              "rotate_all": "rotate_all_kernel"}

@@ -95,12 +95,19 @@ __device__ inline uint32_t unshift(uint32_t X, uint32_t n) {
 // Bilinear sample at the back-mapped point of destination pixel (X, Y) with
 // inverse coefficients inv = (i00, i01, b0, i10, i11, b1).  Returns false if the
 // point is outside [0, nx-1] x [0, ny-1]; taps with zero weight are skipped.
+// Back-mapped source point, evaluated exactly as written: it decides which
+// destination pixels are written (bit-identical to the host definition).
+__device__ inline void back_map_point(const double inv[6], uint32_t X, uint32_t Y, double &sx, double &sy) {
+#pragma clang fp contract(off)
+  sx = inv[0] * X + inv[1] * Y + inv[2];
+  sy = inv[3] * X + inv[4] * Y + inv[5];
+}
+
 template <typename T>
 __device__ inline bool bilinear_sample(const T *src, uint32_t nx, uint32_t ny, const double inv[6], uint32_t X,
                                        uint32_t Y, double *out) {
-#pragma clang fp contract(off)
-  const double sx = inv[0] * X + inv[1] * Y + inv[2];
-  const double sy = inv[3] * X + inv[4] * Y + inv[5];
+  double sx, sy;
+  back_map_point(inv, X, Y, sx, sy);
   const bool inside = sx >= 0 && sx <= (double)(nx - 1) && sy >= 0 && sy <= (double)(ny - 1);
   // outside points sample (0, 0) and are reported invalid: no branch around the loads
   const double fx0 = inside ? floor(sx) : 0.0, fy0 = inside ? floor(sy) : 0.0;

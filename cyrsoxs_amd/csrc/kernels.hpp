@@ -177,8 +177,11 @@ template <typename T> struct MbAnglesArgs {
   uint32_t M;
   const T *amat;                   // [nA][kEbAmat]
   uint32_t nA;
-  T *proj;                         // [nA][Ny][Nx]
+  T *proj;                         // [nA][Ny][Nx] (per-angle projections), or
+  T *quad;                         // [Ny][Nx][4]: laboratory frame, the intensity's quadratic form
+  Mat3<T> Q;                       // frame e1, e2, k^ (quadratic form)
 };
+// quad set: one quadratic form per pixel (laboratory frame), else every angle's projection.
 template <typename T> cyr_status launch_mbasis_angles(const MbAnglesArgs<T> &a, int ng, hipStream_t st);
 
 // ---- rotation / averaging (rotate.hip) --------------------------------------
@@ -193,6 +196,11 @@ cyr_status launch_average(const double *acc, const uint32_t *count, double *avg,
 template <typename T>
 cyr_status launch_rotate_all(const T *proj, const double *inv_tab, uint32_t nA, uint32_t nx, uint32_t ny,
                              int rot_mask, double *avg, hipStream_t st);
+// Laboratory frame: the same from the per-pixel quadratic form quad[Ny][Nx][4]
+// (I(theta) = c^2 q0 + s^2 q1 + 2 c s q2) with coef[a] = (c^2, s^2, 2 c s).
+template <typename T>
+cyr_status launch_rotate_quad(const T *quad, const double *inv_tab, const double *coef, uint32_t nA, uint32_t nx,
+                              uint32_t ny, int rot_mask, double *avg, hipStream_t st);
 // Detector warp of the averaged pattern into the output (prefill 0 or NaN).
 template <typename T>
 cyr_status launch_detector_warp(const double *avg, T *out, uint32_t nx, uint32_t ny, const double inv[6],

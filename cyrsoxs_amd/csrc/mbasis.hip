@@ -220,21 +220,18 @@ __device__ inline void add_basis(int r, const MatConst<T> &mc, const cx<T> &x, c
   }
 }
 
-// ---- per energy: fields at the Ewald bins, every angle's intensity ---------------
+// ---- per energy: the tensor fields at a pixel's Ewald bins ----------------------
+// Returns 0 for a NaN pixel; else fills F[q] (bin z0 + q), the scattering
+// matrices md[q] and the linear weight w1 of bin z0 + 1.
 template <typename T, int NG>
-__global__ void __launch_bounds__(256) mbasis_angles_kernel(MbAnglesArgs<T> a) {
+__device__ inline int mb_fields(const MbAnglesArgs<T> &a, uint32_t X, uint32_t Y, cx<T> (&F)[NG][kEbFields],
+                                double (&md)[NG][6], T &w1) {
   const EwaldGeom &g = a.g;
   const uint64_t P2 = (uint64_t)g.nx * g.ny;
-  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= P2) return;
-  const uint32_t Y = (uint32_t)(t / g.nx), X = (uint32_t)(t - (uint64_t)Y * g.nx);
   uint32_t z0 = 0;
   double f = 0;
   const int kind = ewald_geometry(g, X, Y, &z0, &f);
-  if (kind == 0) {
-    for (uint32_t an = 0; an < a.nA; an++) a.proj[(uint64_t)an * P2 + t] = mb_qnan<T>();
-    return;
-  }
+  if (kind == 0) return 0;
   const uint32_t ix = unshift(X, g.nx), iy = unshift(Y, g.ny);
   const uint64_t col = (uint64_t)iy * g.nx + ix;
   const bool three = !g.enable2d;
@@ -254,7 +251,7 @@ __global__ void __launch_bounds__(256) mbasis_angles_kernel(MbAnglesArgs<T> a) {
       zm = mir[(uint64_t)v * slab + (uint64_t)q * P2];
     }
   };
-  cx<T> F[NG][kEbFields], iso[NG];
+  cx<T> iso[NG];
 #pragma unroll
   for (int q = 0; q < NG; q++) {
     iso[q] = mk<T>(0, 0);
@@ -297,11 +294,26 @@ __global__ void __launch_bounds__(256) mbasis_angles_kernel(MbAnglesArgs<T> a) {
     F[q][0].y += iso[q].y;
     F[q][3].x += iso[q].x;
     F[q][3].y += iso[q].y;
+    scatter_matrix(g, X, Y, three ? z0 + q : 0, md[q]);
   }
+  w1 = (T)f;
+  return kind;
+}
+
+// ---- per energy: every angle's projection ---------------------------------------
+template <typename T, int NG>
+__global__ void __launch_bounds__(256) mbasis_angles_kernel(MbAnglesArgs<T> a) {
+  const uint64_t P2 = (uint64_t)a.g.nx * a.g.ny;
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= P2) return;
+  const uint32_t Y = (uint32_t)(t / a.g.nx), X = (uint32_t)(t - (uint64_t)Y * a.g.nx);
+  cx<T> F[NG][kEbFields];
   double md[NG][6];
-#pragma unroll
-  for (int q = 0; q < NG; q++) scatter_matrix(g, X, Y, three ? z0 + q : 0, md[q]);
-  const T w1 = (T)f;
+  T w1;
+  if (!mb_fields<T, NG>(a, X, Y, F, md, w1)) {
+    for (uint32_t an = 0; an < a.nA; an++) a.proj[(uint64_t)an * P2 + t] = mb_qnan<T>();
+    return;
+  }
 #pragma unroll 4
   for (uint32_t an = 0; an < a.nA; an++) {
     const T *A = a.amat + (uint64_t)kEbAmat * an;
@@ -326,6 +338,67 @@ __global__ void __launch_bounds__(256) mbasis_angles_kernel(MbAnglesArgs<T> a) {
     else out = I[0];
     a.proj[(uint64_t)an * P2 + t] = out;
   }
+}
+
+// ---- per energy, laboratory frame: the intensity's quadratic form ---------------
+// With E(theta) = c e1 + s e2 the polarization is p = c u + s v, u = Q (F0, F1, F2),
+// v = Q (F1, F3, F4), so I(theta) = c^2 |M u|^2 + s^2 |M v|^2 + 2 c s Re<M u, M v>.
+// quad[pixel] = (|Mu|^2, |Mv|^2, Re<Mu, Mv>, 0), interpolated between the bins.
+template <typename T, int NG>
+__global__ void __launch_bounds__(256) mbasis_quad_kernel(MbAnglesArgs<T> a) {
+  const uint64_t P2 = (uint64_t)a.g.nx * a.g.ny;
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= P2) return;
+  const uint32_t Y = (uint32_t)(t / a.g.nx), X = (uint32_t)(t - (uint64_t)Y * a.g.nx);
+  cx<T> F[NG][kEbFields];
+  double md[NG][6];
+  T w1;
+  T *o = a.quad + 4 * t;
+  if (!mb_fields<T, NG>(a, X, Y, F, md, w1)) {
+    o[0] = o[1] = o[2] = mb_qnan<T>();
+    o[3] = 0;
+    return;
+  }
+  const Mat3<T> &Q = a.Q;
+  T qa[NG], qb[NG], qm[NG];
+#pragma unroll
+  for (int q = 0; q < NG; q++) {
+    cx<T> u[3], v[3], mu[3], mv[3];
+    const cx<T> ue[3] = {F[q][0], F[q][1], F[q][2]}, ve[3] = {F[q][1], F[q][3], F[q][4]};
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      u[i] = mk<T>(Q.m[3 * i] * ue[0].x + Q.m[3 * i + 1] * ue[1].x + Q.m[3 * i + 2] * ue[2].x,
+                   Q.m[3 * i] * ue[0].y + Q.m[3 * i + 1] * ue[1].y + Q.m[3 * i + 2] * ue[2].y);
+      v[i] = mk<T>(Q.m[3 * i] * ve[0].x + Q.m[3 * i + 1] * ve[1].x + Q.m[3 * i + 2] * ve[2].x,
+                   Q.m[3 * i] * ve[0].y + Q.m[3 * i + 1] * ve[1].y + Q.m[3 * i + 2] * ve[2].y);
+    }
+    const T m0 = (T)md[q][0], m1 = (T)md[q][1], m2 = (T)md[q][2], m3 = (T)md[q][3], m4 = (T)md[q][4],
+            m5 = (T)md[q][5];
+    const T mm[3][3] = {{m0, m1, m2}, {m1, m3, m4}, {m2, m4, m5}};
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      mu[i] = mk<T>(mm[i][0] * u[0].x + mm[i][1] * u[1].x + mm[i][2] * u[2].x,
+                    mm[i][0] * u[0].y + mm[i][1] * u[1].y + mm[i][2] * u[2].y);
+      mv[i] = mk<T>(mm[i][0] * v[0].x + mm[i][1] * v[1].x + mm[i][2] * v[2].x,
+                    mm[i][0] * v[0].y + mm[i][1] * v[1].y + mm[i][2] * v[2].y);
+    }
+    qa[q] = mu[0].x * mu[0].x + mu[0].y * mu[0].y + mu[1].x * mu[1].x + mu[1].y * mu[1].y + mu[2].x * mu[2].x +
+            mu[2].y * mu[2].y;
+    qb[q] = mv[0].x * mv[0].x + mv[0].y * mv[0].y + mv[1].x * mv[1].x + mv[1].y * mv[1].y + mv[2].x * mv[2].x +
+            mv[2].y * mv[2].y;
+    qm[q] = mu[0].x * mv[0].x + mu[0].y * mv[0].y + mu[1].x * mv[1].x + mu[1].y * mv[1].y + mu[2].x * mv[2].x +
+            mu[2].y * mv[2].y;
+  }
+  if constexpr (NG == 2) {
+    o[0] = (1 - w1) * qa[0] + w1 * qa[1];
+    o[1] = (1 - w1) * qb[0] + w1 * qb[1];
+    o[2] = (1 - w1) * qm[0] + w1 * qm[1];
+  } else {
+    o[0] = qa[0];
+    o[1] = qb[0];
+    o[2] = qm[0];
+  }
+  o[3] = 0;
 }
 
 }  // namespace
@@ -391,8 +464,13 @@ template <typename T> cyr_status launch_mbasis_zband(const BandArgs<T> &a, hipSt
 template <typename T> cyr_status launch_mbasis_angles(const MbAnglesArgs<T> &a, int ng, hipStream_t st) {
   const uint64_t P = (uint64_t)a.g.nx * a.g.ny;
   const dim3 grid(ceil_div(P, 256)), block(256);
-  if (ng == 2) hipLaunchKernelGGL((mbasis_angles_kernel<T, 2>), grid, block, 0, st, a);
-  else hipLaunchKernelGGL((mbasis_angles_kernel<T, 1>), grid, block, 0, st, a);
+  if (a.quad) {
+    if (ng == 2) hipLaunchKernelGGL((mbasis_quad_kernel<T, 2>), grid, block, 0, st, a);
+    else hipLaunchKernelGGL((mbasis_quad_kernel<T, 1>), grid, block, 0, st, a);
+  } else {
+    if (ng == 2) hipLaunchKernelGGL((mbasis_angles_kernel<T, 2>), grid, block, 0, st, a);
+    else hipLaunchKernelGGL((mbasis_angles_kernel<T, 1>), grid, block, 0, st, a);
+  }
   CYR_LAUNCHED();
   return CYR_OK;
 }

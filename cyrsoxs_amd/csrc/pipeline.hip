@@ -78,7 +78,7 @@ template <typename T> MatConst<T> mat_const(const double *m4) {
 
 // Per-simulation constant tables, uploaded once.
 struct Tables {
-  size_t mats = 0, han = 0, amat = 0, rotinv = 0, detinv = 0, kmag = 0, total = 0;
+  size_t mats = 0, han = 0, amat = 0, rotinv = 0, detinv = 0, kmag = 0, quad = 0, total = 0;
   std::vector<unsigned char> host;
 };
 
@@ -141,6 +141,8 @@ template <typename T> void build_layout(const SimGeometry &g, Tables &tb) {
   off = align256(off + sizeof(double) * 6 * g.nK);
   tb.kmag = off;
   off = align256(off + sizeof(double) * g.nE);
+  tb.quad = off;
+  off = align256(off + sizeof(double) * 3 * g.nK * g.nA);
   tb.total = off;
 }
 
@@ -167,6 +169,14 @@ void build_tables(const cyr_sim_desc *d, const SimGeometry &g, const double *mat
   std::memcpy(tb.host.data() + tb.rotinv, g.rot_inv.data(), sizeof(double) * 6 * g.nK * g.nA);
   std::memcpy(tb.host.data() + tb.detinv, g.det_inv.data(), sizeof(double) * 6 * g.nK);
   std::memcpy(tb.host.data() + tb.kmag, g.kmag.data(), sizeof(double) * g.nE);
+  // laboratory frame: I(theta) = c^2 q0 + s^2 q1 + 2 c s q2 with E = c e1 + s e2
+  double *qc = reinterpret_cast<double *>(tb.host.data() + tb.quad);
+  for (size_t i = 0; i < (size_t)g.nK * g.nA; i++) {
+    const double c = g.ebasis[2 * i], sn = g.ebasis[2 * i + 1];
+    qc[3 * i + 0] = c * c;
+    qc[3 * i + 1] = sn * sn;
+    qc[3 * i + 2] = 2.0 * c * sn;
+  }
 }
 
 template <typename T> Vec3<T> v3(const V3 &v) { return Vec3<T>{(T)v.x, (T)v.y, (T)v.z}; }
@@ -180,7 +190,7 @@ template <typename T> Mat3<T> m3(const M3 &m) {
 template <typename T> struct Setup {
   const MatConst<T> *mats;
   const T *hx, *hy, *hz, *amat;
-  const double *rotinv, *kmag;
+  const double *rotinv, *kmag, *quad;
   const cx<T> *twy, *twz;
   double vox_bytes, cxb;
 };
@@ -200,6 +210,7 @@ cyr_status setup_views(const cyr_sim_desc *d, const SimGeometry &g, const Tables
   su.amat = reinterpret_cast<const T *>(tbase + tb.amat);
   su.rotinv = reinterpret_cast<const double *>(tbase + tb.rotinv);
   su.kmag = reinterpret_cast<const double *>(tbase + tb.kmag);
+  su.quad = reinterpret_cast<const double *>(tbase + tb.quad);
   cyr_status s;
   su.twz = g.nz > 1 ? twiddles<T>(g.nz, &s) : nullptr;
   if (g.nz > 1 && !su.twz) return s;
@@ -473,9 +484,11 @@ cyr_status run_mbasis(const cyr_sim_desc *d, const SimGeometry &g, const void *d
   const bool three = g.nz > 1;
   const int ng = (three && d->interpolation == 1) ? 2 : 1;
   const uint32_t nvol = mbasis_volumes(g.M), pass_max = nvol < 7 ? nvol : 7;
+  // laboratory frame: one quadratic form per pixel instead of every angle's projection
+  const bool lab = d->reference_frame != 0;
   LaneBuffers &lb = wb.lane[0];
   CYR_TRY(ensure_buffer(lb.pol, sizeof(cx<T>) * V * pass_max));
-  CYR_TRY(ensure_buffer(lb.proj, sizeof(T) * P * g.nA));
+  CYR_TRY(ensure_buffer(lb.proj, sizeof(T) * P * (lab ? 4 : g.nA)));
   CYR_TRY(ensure_buffer(lb.avg, sizeof(double) * P));
   CYR_TRY(ensure_buffer(lb.bins, sizeof(int32_t) * P + sizeof(uint32_t) * mbasis_bmax_words(g.nx, g.ny)));
   CYR_TRY(ensure_buffer(lb.dc, sizeof(double2) * nvol * (g.nz + 1)));
@@ -591,13 +604,26 @@ cyr_status run_mbasis(const cyr_sim_desc *d, const SimGeometry &g, const void *d
         aa.M = g.M;
         aa.amat = su.amat + (size_t)kEbAmat * k * g.nA;
         aa.nA = g.nA;
-        aa.proj = proj;
-        kt_begin(s);
-        CYR_TRY(launch_mbasis_angles<T>(aa, ng, s));
-        kt_end("mbasis_angles", s, 2.0 * ng * nvol * (double)P * cxb + (double)g.nA * P * sizeof(T));
-        kt_begin(s);
-        CYR_TRY(launch_rotate_all<T>(proj, su.rotinv + (size_t)6 * k * g.nA, g.nA, g.nx, g.ny, d->rot_mask, avg, s));
-        kt_end("rotate_all", s, (double)(g.nA + 1) * P * sizeof(T));
+        aa.Q = m3<T>(Q);
+        if (lab) aa.quad = proj;
+        else aa.proj = proj;
+        const double *inv = su.rotinv + (size_t)6 * k * g.nA;
+        if (lab) {
+          kt_begin(s);
+          CYR_TRY(launch_mbasis_angles<T>(aa, ng, s));
+          kt_end("mbasis_quad", s, 2.0 * ng * nvol * (double)P * cxb + 4.0 * P * sizeof(T));
+          kt_begin(s);
+          CYR_TRY(launch_rotate_quad<T>(proj, inv, su.quad + (size_t)3 * k * g.nA, g.nA, g.nx, g.ny, d->rot_mask, avg,
+                                        s));
+          kt_end("rotate_quad", s, 4.0 * P * sizeof(T) + 8.0 * P);
+        } else {
+          kt_begin(s);
+          CYR_TRY(launch_mbasis_angles<T>(aa, ng, s));
+          kt_end("mbasis_angles", s, 2.0 * ng * nvol * (double)P * cxb + (double)g.nA * P * sizeof(T));
+          kt_begin(s);
+          CYR_TRY(launch_rotate_all<T>(proj, inv, g.nA, g.nx, g.ny, d->rot_mask, avg, s));
+          kt_end("rotate_all", s, (double)(g.nA + 1) * P * sizeof(T));
+        }
         T *out = d_out + ((uint64_t)e * g.nK + k) * P;
         CYR_TRY(launch_detector_warp<T>(avg, out, g.nx, g.ny, &g.det_inv[(size_t)k * 6], d->rot_mask, s));
       }
@@ -761,8 +787,9 @@ uint64_t work_bytes(const cyr_sim_desc *d, const SimGeometry &g) {
     // band table, DC sums, per-angle projections, average
     const uint64_t nv = mbasis_volumes(g.M), pass = nv < 7 ? nv : 7;
     const uint64_t J = mbasis_band_cap(g) > 2 ? mbasis_band_cap(g) : 2;
+    const uint64_t planes = d->reference_frame != 0 ? 4 : g.nA;
     b += cs * g.V * pass + cs * 2 * nv * J * g.P + 4 * g.P + 4 * mbasis_bmax_words(g.nx, g.ny) +
-         16 * nv * (g.nz + 1) + rs * g.P * g.nA + 8 * g.P;
+         16 * nv * (g.nz + 1) + rs * g.P * planes + 8 * g.P;
   } else if (fused) {  // per lane: fields, bin table, Y/z partial sums (chunks x 2 bins x fields), DC sums,
                  // per-angle projections, average
     const uint64_t nch = ebasis_max_chunks(g.nx, g.ny, g.nz, d->precision);

@@ -10,6 +10,8 @@
 // [0, Nx-1] x [0, Ny-1]; taps of exactly zero weight are skipped.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "geom.hpp"
 #include "kernels.hpp"
 
@@ -18,6 +20,17 @@ namespace cyr {
 namespace {
 
 struct Inv6 { double c[6]; };
+
+// Pixel of this thread in the rotation kernels: 256-thread blocks cover 16 x 16
+// tiles, each wave an 8 x 8 quadrant, so a wave's rotated source footprint is
+// compact (a 64-pixel row segment maps to a 64-pixel line of source rows at
+// most angles, touching many more cache lines).
+__device__ inline bool tile_pixel(uint32_t nx, uint32_t ny, uint32_t &X, uint32_t &Y) {
+  const uint32_t w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  X = blockIdx.x * 16 + (w & 1) * 8 + (l & 7);
+  Y = blockIdx.y * 16 + (w >> 1) * 8 + (l >> 3);
+  return X < nx && Y < ny;
+}
 
 template <typename T>
 __global__ void __launch_bounds__(256) rotate_accumulate_kernel(const T *__restrict__ proj, double *__restrict__ acc,
@@ -55,9 +68,9 @@ __global__ void __launch_bounds__(256) rotate_all_kernel(const T *__restrict__ p
                                                          uint32_t nA, uint32_t nx, uint32_t ny, int rot_mask,
                                                          double *__restrict__ avg) {
   const uint64_t P = (uint64_t)nx * ny;
-  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= P) return;
-  const uint32_t Y = (uint32_t)(t / nx), X = (uint32_t)(t - (uint64_t)Y * nx);
+  uint32_t X, Y;
+  if (!tile_pixel(nx, ny, X, Y)) return;
+  const uint64_t t = (uint64_t)Y * nx + X;
   double acc = 0;
   uint32_t cnt = 0;
   // independent gathers per angle; unrolling lets several be in flight (the sum
@@ -74,6 +87,58 @@ __global__ void __launch_bounds__(256) rotate_all_kernel(const T *__restrict__ p
         acc += v;
         cnt++;
       }
+    } else {
+      acc += v;
+    }
+  }
+  if (rot_mask) avg[t] = cnt == 0 ? 0.0 : acc * (1.0 / (cnt * 1.0));
+  else avg[t] = acc * (1.0 / (double)nA);
+}
+
+// Laboratory frame: every angle's intensity from the per-pixel quadratic form
+// q = (q0, q1, q2): I(theta) = c^2 q0 + s^2 q1 + 2 c s q2.  Bilinear sampling is
+// linear, so the three components are sampled (same taps, weights and validity
+// as bilinear_sample; a tap of zero weight contributes nothing, NaN included)
+// and combined once per angle.
+template <typename T>
+__global__ void __launch_bounds__(256) rotate_quad_kernel(const T *__restrict__ quad, const double *__restrict__ inv_tab,
+                                                          const double *__restrict__ coef, uint32_t nA, uint32_t nx,
+                                                          uint32_t ny, int rot_mask, double *__restrict__ avg) {
+  uint32_t X, Y;
+  if (!tile_pixel(nx, ny, X, Y)) return;
+  const uint64_t t = (uint64_t)Y * nx + X;
+  using V4 = typename std::conditional<sizeof(T) == 4, float4, double4>::type;
+  const V4 *q4 = reinterpret_cast<const V4 *>(quad);
+  double acc = 0;
+  uint32_t cnt = 0;
+#pragma unroll 4
+  for (uint32_t a = 0; a < nA; a++) {
+    double sx, sy;
+    back_map_point(inv_tab + 6 * (uint64_t)a, X, Y, sx, sy);
+    const bool inside = sx >= 0 && sx <= (double)(nx - 1) && sy >= 0 && sy <= (double)(ny - 1);
+    const double fx0 = inside ? floor(sx) : 0.0, fy0 = inside ? floor(sy) : 0.0;
+    const uint32_t x0 = (uint32_t)fx0, y0 = (uint32_t)fy0;
+    const double fx = inside ? sx - fx0 : 0.0, fy = inside ? sy - fy0 : 0.0;
+    const uint32_t x1 = x0 + 1 < nx ? x0 + 1 : x0, y1 = y0 + 1 < ny ? y0 + 1 : y0;
+    const uint32_t r0 = y0 * nx, r1 = y1 * nx;  // P < 2^32
+    const V4 a00 = q4[r0 + x0], a10 = q4[r0 + x1], a01 = q4[r1 + x0], a11 = q4[r1 + x1];
+    const double gx = 1 - fx, gy = 1 - fy;
+    const double w[4] = {gx * gy, fx * gy, gx * fy, fx * fy};
+    const V4 tap[4] = {a00, a10, a01, a11};
+    double s0 = 0, s1 = 0, s2 = 0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const bool z = w[i] == 0;  // selected in the storage type: one move per component
+      s0 += w[i] * (double)(z ? (T)0 : tap[i].x);
+      s1 += w[i] * (double)(z ? (T)0 : tap[i].y);
+      s2 += w[i] * (double)(z ? (T)0 : tap[i].z);
+    }
+    const double *c = coef + 3 * (uint64_t)a;
+    const double v = inside ? c[0] * s0 + c[1] * s1 + c[2] * s2 : __longlong_as_double(0x7ff8000000000000ll);
+    if (rot_mask) {
+      const bool ok = !isnan(v);
+      acc += ok ? v : 0.0;
+      cnt += ok ? 1u : 0u;
     } else {
       acc += v;
     }
@@ -133,8 +198,20 @@ template <typename T>
 cyr_status launch_rotate_all(const T *proj, const double *inv_tab, uint32_t nA, uint32_t nx, uint32_t ny, int rot_mask,
                              double *avg, hipStream_t st) {
   const uint64_t P = (uint64_t)nx * ny;
-  hipLaunchKernelGGL((rotate_all_kernel<T>), dim3(ceil_div(P, 256)), dim3(256), 0, st, proj, inv_tab, nA, nx, ny,
-                     rot_mask, avg);
+  (void)P;
+  hipLaunchKernelGGL((rotate_all_kernel<T>), dim3(ceil_div(nx, 16), ceil_div(ny, 16)), dim3(256), 0, st, proj,
+                     inv_tab, nA, nx, ny, rot_mask, avg);
+  CYR_LAUNCHED();
+  return CYR_OK;
+}
+
+template <typename T>
+cyr_status launch_rotate_quad(const T *quad, const double *inv_tab, const double *coef, uint32_t nA, uint32_t nx,
+                              uint32_t ny, int rot_mask, double *avg, hipStream_t st) {
+  const uint64_t P = (uint64_t)nx * ny;
+  (void)P;
+  hipLaunchKernelGGL((rotate_quad_kernel<T>), dim3(ceil_div(nx, 16), ceil_div(ny, 16)), dim3(256), 0, st, quad,
+                     inv_tab, coef, nA, nx, ny, rot_mask, avg);
   CYR_LAUNCHED();
   return CYR_OK;
 }
@@ -162,6 +239,8 @@ cyr_status launch_warp(const T *src, T *dst, uint32_t nx, uint32_t ny, const dou
                                                   const double *, int, hipStream_t);                               \
   template cyr_status launch_rotate_all<T>(const T *, const double *, uint32_t, uint32_t, uint32_t, int, double *, \
                                            hipStream_t);                                                           \
+  template cyr_status launch_rotate_quad<T>(const T *, const double *, const double *, uint32_t, uint32_t,        \
+                                            uint32_t, int, double *, hipStream_t);                                 \
   template cyr_status launch_detector_warp<T>(const double *, T *, uint32_t, uint32_t, const double *, int,        \
                                               hipStream_t);                                                        \
   template cyr_status launch_warp<T>(const T *, T *, uint32_t, uint32_t, const double *, hipStream_t);
