@@ -168,10 +168,6 @@ __global__ void __launch_bounds__(WG) fft_ct_kernel(PassArgs A, const cx<T> *__r
   } else {
     if (xcd_tile(blockIdx.x, gridDim.x) >= A.groups * A.outer_count * A.nvol)
       return;  // grid padding: the whole block leaves before any barrier
-    cx<T> *ltw = lds + (size_t)A.pp * N;
-    for (uint32_t i = threadIdx.x; i < (uint32_t)N; i += WG) ltw[i] = tw[i];
-    lds_sync();
-    twp = ltw;
     p = threadIdx.x % A.pp;
     t = threadIdx.x / A.pp;
   }
@@ -192,6 +188,13 @@ __global__ void __launch_bounds__(WG) fft_ct_kernel(PassArgs A, const cx<T> *__r
   cx<T> v[E];
 #pragma unroll
   for (int i = 0; i < E; i++) v[i] = active ? gbase[(uint64_t)F::in_pos(t, i) * A.elem_stride] : mk<T>(0, 0);
+  if constexpr (!CONTIG) {
+    // twiddles staged while the pencil loads are in flight
+    cx<T> *ltw = lds + (size_t)A.pp * N;
+    for (uint32_t i = threadIdx.x; i < (uint32_t)N; i += WG) ltw[i] = tw[i];
+    lds_sync();
+    twp = ltw;
+  }
   F::run(v, t, p, lds, twp);
   if (active) {
 #pragma unroll

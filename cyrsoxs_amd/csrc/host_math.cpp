@@ -1,7 +1,10 @@
 // FP64 host geometry (see host_math.hpp).  Compiled with -ffp-contract=off.
 #include "host_math.hpp"
 
+#include <array>
 #include <cmath>
+#include <map>
+#include <mutex>
 #include <cstring>
 #include <limits>
 #include <string>
@@ -77,18 +80,41 @@ V3 rodrigues(const V3 &v, const V3 &axis, double angle) {
 
 M3 rotation_matrix_k(const V3 &k) { return rotation_vec_to_vec(V3{0, 0, 1}, k); }
 
+namespace {
+// The search below is a pure function of (k, RK) and costs ~1 ms of host time;
+// every simulation with the same k vector reuses its result.
+std::mutex g_base_mu;
+std::map<std::array<double, 12>, double> g_base_search;
+}  // namespace
+
 double base_configuration(const V3 &k, const M3 &RK, M3 *R) {
-  const unsigned num_interval = 100000;
-  const double dtheta = M_PI / (num_interval * 1.0);
+  std::array<double, 12> key{k.x, k.y, k.z};
+  for (int i = 0; i < 9; i++) key[3 + i] = RK.m[i];
   const V3 sx = matvec(RK, V3{1, 0, 0});
-  double maxdiff = std::numeric_limits<double>::infinity(), ang = 0;
-  for (unsigned i = 0; i < num_interval; i++) {
-    const V3 rx = rodrigues(sx, k, i * dtheta);
-    const double diff = std::fabs(rx.y);
-    if (maxdiff > diff) {
-      maxdiff = diff;
-      ang = i * dtheta;
+  double ang = 0;
+  bool cached = false;
+  {
+    std::lock_guard<std::mutex> lk(g_base_mu);
+    auto it = g_base_search.find(key);
+    if (it != g_base_search.end()) {
+      ang = it->second;
+      cached = true;
     }
+  }
+  if (!cached) {
+    const unsigned num_interval = 100000;
+    const double dtheta = M_PI / (num_interval * 1.0);
+    double maxdiff = std::numeric_limits<double>::infinity();
+    for (unsigned i = 0; i < num_interval; i++) {
+      const V3 rx = rodrigues(sx, k, i * dtheta);
+      const double diff = std::fabs(rx.y);
+      if (maxdiff > diff) {
+        maxdiff = diff;
+        ang = i * dtheta;
+      }
+    }
+    std::lock_guard<std::mutex> lk(g_base_mu);
+    g_base_search[key] = ang;
   }
   V3 rx = rodrigues(sx, k, ang);
   ang = rx.x > 0 ? ang : ang + M_PI;

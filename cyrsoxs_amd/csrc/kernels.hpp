@@ -178,10 +178,13 @@ template <typename T> struct MbAnglesArgs {
   const T *amat;                   // [nA][kEbAmat]
   uint32_t nA;
   T *proj;                         // [nA][Ny][Nx] (per-angle projections), or
-  T *quad;                         // [Ny][Nx][4]: laboratory frame, the intensity's quadratic form
+  T *quad;                         // [ne][Ny][Nx][4]: laboratory frame, the intensity's quadratic form
   Mat3<T> Q;                       // frame e1, e2, k^ (quadratic form)
+  const double *kmag;              // quadratic form: |k| of the ne energies (mats then holds ne x M)
+  uint32_t ne;
 };
-// quad set: one quadratic form per pixel (laboratory frame), else every angle's projection.
+// quad set: one quadratic form per pixel for each of ne energies in one launch
+// (laboratory frame), else every angle's projection of one energy.
 template <typename T> cyr_status launch_mbasis_angles(const MbAnglesArgs<T> &a, int ng, hipStream_t st);
 
 // ---- rotation / averaging (rotate.hip) --------------------------------------
@@ -192,19 +195,30 @@ cyr_status launch_rotate_accumulate(const T *proj, double *acc, uint32_t *count,
 // Average over angles into avg (FP64): mask division or 1/nA.
 cyr_status launch_average(const double *acc, const uint32_t *count, double *avg, uint64_t P, uint32_t nA,
                           int rot_mask, hipStream_t st);
-// E-basis path: rotate every angle's projection, accumulate and average in one pass.
+// Output tiles (16 x 16) of the rotation kernels, ordered by distance from the
+// rotation centre (ty << 16 | tx); the kernels give XCD x the x-th annulus.
+uint32_t rotation_tile_count(uint32_t nx, uint32_t ny);
+void rotation_tile_order(uint32_t nx, uint32_t ny, uint32_t *out);
+// E-basis path: rotate every angle's projection, accumulate and average in one pass
+// (tiles: rotation_tile_order on the device).
 template <typename T>
 cyr_status launch_rotate_all(const T *proj, const double *inv_tab, uint32_t nA, uint32_t nx, uint32_t ny,
-                             int rot_mask, double *avg, hipStream_t st);
-// Laboratory frame: the same from the per-pixel quadratic form quad[Ny][Nx][4]
-// (I(theta) = c^2 q0 + s^2 q1 + 2 c s q2) with coef[a] = (c^2, s^2, 2 c s).
+                             int rot_mask, double *avg, const uint32_t *tiles, hipStream_t st);
+// Laboratory frame: the same for ne energies from their per-pixel quadratic forms
+// quad[ne][Ny][Nx][4] (I(theta) = c^2 q0 + s^2 q1 + 2 c s q2) with coef[a] =
+// (c^2, s^2, 2 c s) into avg[ne][Ny][Nx]; each angle's taps serve several energies.
 template <typename T>
 cyr_status launch_rotate_quad(const T *quad, const double *inv_tab, const double *coef, uint32_t nA, uint32_t nx,
-                              uint32_t ny, int rot_mask, double *avg, hipStream_t st);
+                              uint32_t ny, int rot_mask, double *avg, uint32_t ne, const uint32_t *tiles,
+                              hipStream_t st);
 // Detector warp of the averaged pattern into the output (prefill 0 or NaN).
 template <typename T>
 cyr_status launch_detector_warp(const double *avg, T *out, uint32_t nx, uint32_t ny, const double inv[6],
                                 int rot_mask, hipStream_t st);
+// The same for ne averages avg[ne][Ny][Nx] into out + e * out_stride.
+template <typename T>
+cyr_status launch_detector_warp_n(const double *avg, T *out, uint64_t out_stride, uint32_t ne, uint32_t nx,
+                                  uint32_t ny, const double inv[6], int rot_mask, hipStream_t st);
 // Generic bilinear warp (stage-level entry point).
 template <typename T>
 cyr_status launch_warp(const T *src, T *dst, uint32_t nx, uint32_t ny, const double inv[6], hipStream_t st);

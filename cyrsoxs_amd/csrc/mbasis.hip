@@ -119,9 +119,6 @@ __global__ void __launch_bounds__(WG) mbasis_zband_kernel(BandArgs<T> a, const c
   const uint32_t groups = (a.nx + PP - 1) / PP, per_vol = groups * a.ny;
   const uint32_t q = xcd_tile(blockIdx.x, gridDim.x);
   if (q >= per_vol * a.count) return;  // grid padding: the whole block leaves before any barrier
-  cx<T> *ltw = lds + (size_t)PP * N;
-  for (uint32_t i = threadIdx.x; i < (uint32_t)N; i += WG) ltw[i] = tw[i];
-  lds_sync();
   const uint32_t p = threadIdx.x % PP, t = threadIdx.x / PP;
   const uint32_t v = q / per_vol, tile = q % per_vol;
   const uint32_t g = tile % groups, y = tile / groups;
@@ -130,9 +127,13 @@ __global__ void __launch_bounds__(WG) mbasis_zband_kernel(BandArgs<T> a, const c
   const uint64_t P = (uint64_t)a.nx * a.ny;
   const uint64_t col = (uint64_t)y * a.nx + (active ? x : 0);
   const cx<T> *base = a.vol[v] + col;
+  // pencil loads first: they stay in flight across the twiddle staging barrier
   cx<T> r[E];
 #pragma unroll
   for (int i = 0; i < E; i++) r[i] = base[(uint64_t)F::in_pos(t, i) * P];
+  cx<T> *ltw = lds + (size_t)PP * N;
+  for (uint32_t i = threadIdx.x; i < (uint32_t)N; i += WG) ltw[i] = tw[i];
+  lds_sync();
   F::run(r, t, p, lds, ltw);
   if (!active) return;
   const uint32_t b = (uint32_t)a.band[col];
@@ -349,11 +350,14 @@ __global__ void __launch_bounds__(256) mbasis_quad_kernel(MbAnglesArgs<T> a) {
   const uint64_t P2 = (uint64_t)a.g.nx * a.g.ny;
   const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= P2) return;
+  // grid y = energy of the group: its |k|, constants and output plane
+  a.g.k = a.kmag[blockIdx.y];
+  a.mats += (uint64_t)blockIdx.y * a.M;
   const uint32_t Y = (uint32_t)(t / a.g.nx), X = (uint32_t)(t - (uint64_t)Y * a.g.nx);
   cx<T> F[NG][kEbFields];
   double md[NG][6];
   T w1;
-  T *o = a.quad + 4 * t;
+  T *o = a.quad + ((uint64_t)blockIdx.y * P2 + t) * 4;
   if (!mb_fields<T, NG>(a, X, Y, F, md, w1)) {
     o[0] = o[1] = o[2] = mb_qnan<T>();
     o[3] = 0;
@@ -465,8 +469,10 @@ template <typename T> cyr_status launch_mbasis_angles(const MbAnglesArgs<T> &a, 
   const uint64_t P = (uint64_t)a.g.nx * a.g.ny;
   const dim3 grid(ceil_div(P, 256)), block(256);
   if (a.quad) {
-    if (ng == 2) hipLaunchKernelGGL((mbasis_quad_kernel<T, 2>), grid, block, 0, st, a);
-    else hipLaunchKernelGGL((mbasis_quad_kernel<T, 1>), grid, block, 0, st, a);
+    if (!a.kmag || a.ne == 0 || a.ne > 65535) return fail(CYR_E_INVALID, "quadratic form: energy table missing");
+    const dim3 g2(ceil_div(P, 256), a.ne);
+    if (ng == 2) hipLaunchKernelGGL((mbasis_quad_kernel<T, 2>), g2, block, 0, st, a);
+    else hipLaunchKernelGGL((mbasis_quad_kernel<T, 1>), g2, block, 0, st, a);
   } else {
     if (ng == 2) hipLaunchKernelGGL((mbasis_angles_kernel<T, 2>), grid, block, 0, st, a);
     else hipLaunchKernelGGL((mbasis_angles_kernel<T, 1>), grid, block, 0, st, a);

@@ -78,7 +78,7 @@ template <typename T> MatConst<T> mat_const(const double *m4) {
 
 // Per-simulation constant tables, uploaded once.
 struct Tables {
-  size_t mats = 0, han = 0, amat = 0, rotinv = 0, detinv = 0, kmag = 0, quad = 0, total = 0;
+  size_t mats = 0, han = 0, amat = 0, rotinv = 0, detinv = 0, kmag = 0, quad = 0, tiles = 0, total = 0;
   std::vector<unsigned char> host;
 };
 
@@ -143,6 +143,8 @@ template <typename T> void build_layout(const SimGeometry &g, Tables &tb) {
   off = align256(off + sizeof(double) * g.nE);
   tb.quad = off;
   off = align256(off + sizeof(double) * 3 * g.nK * g.nA);
+  tb.tiles = off;
+  off = align256(off + sizeof(uint32_t) * rotation_tile_count(g.nx, g.ny));
   tb.total = off;
 }
 
@@ -177,6 +179,7 @@ void build_tables(const cyr_sim_desc *d, const SimGeometry &g, const double *mat
     qc[3 * i + 1] = sn * sn;
     qc[3 * i + 2] = 2.0 * c * sn;
   }
+  rotation_tile_order(g.nx, g.ny, reinterpret_cast<uint32_t *>(tb.host.data() + tb.tiles));
 }
 
 template <typename T> Vec3<T> v3(const V3 &v) { return Vec3<T>{(T)v.x, (T)v.y, (T)v.z}; }
@@ -191,6 +194,7 @@ template <typename T> struct Setup {
   const MatConst<T> *mats;
   const T *hx, *hy, *hz, *amat;
   const double *rotinv, *kmag, *quad;
+  const uint32_t *tiles;
   const cx<T> *twy, *twz;
   double vox_bytes, cxb;
 };
@@ -211,6 +215,7 @@ cyr_status setup_views(const cyr_sim_desc *d, const SimGeometry &g, const Tables
   su.rotinv = reinterpret_cast<const double *>(tbase + tb.rotinv);
   su.kmag = reinterpret_cast<const double *>(tbase + tb.kmag);
   su.quad = reinterpret_cast<const double *>(tbase + tb.quad);
+  su.tiles = reinterpret_cast<const uint32_t *>(tbase + tb.tiles);
   cyr_status s;
   su.twz = g.nz > 1 ? twiddles<T>(g.nz, &s) : nullptr;
   if (g.nz > 1 && !su.twz) return s;
@@ -416,7 +421,8 @@ cyr_status run_ebasis(const cyr_sim_desc *d, const SimGeometry &g, const void *d
     CYR_TRY(launch_ebasis_angles<T>(aa, ng, s));
     kt_end("ebasis_angles", s, part_bytes + (double)g.nA * P * sizeof(T));
     kt_begin(s);
-    CYR_TRY(launch_rotate_all<T>(proj, su.rotinv + (size_t)6 * k * g.nA, g.nA, g.nx, g.ny, d->rot_mask, avg, s));
+    CYR_TRY(launch_rotate_all<T>(proj, su.rotinv + (size_t)6 * k * g.nA, g.nA, g.nx, g.ny, d->rot_mask, avg,
+                                 su.tiles, s));
     kt_end("rotate_all", s, (double)(g.nA + 1) * P * sizeof(T));
     CYR_TRY(launch_detector_warp<T>(avg, out, g.nx, g.ny, &g.det_inv[(size_t)k * 6], d->rot_mask, s));
   }
@@ -462,6 +468,15 @@ cyr_status mbasis_band(const cyr_sim_desc *d, const SimGeometry &g, uint32_t k, 
   return CYR_OK;
 }
 
+// Energies whose quadratic forms and averages are batched per launch (laboratory
+// frame): up to 32, and no more than ~1 GiB of planes.
+uint32_t mbasis_energy_batch(const SimGeometry &g, size_t real_bytes) {
+  const uint64_t per = (uint64_t)g.P * (4 * real_bytes + 8);
+  uint64_t nb = (1ull << 30) / (per ? per : 1);
+  if (nb < 1) nb = 1;
+  return (uint32_t)(nb > 32 ? 32 : nb);
+}
+
 // Largest band kept per energy group: the band values then take no more memory
 // than the seven transformed volumes of a material pass.
 uint32_t mbasis_band_cap(const SimGeometry &g) {
@@ -486,10 +501,11 @@ cyr_status run_mbasis(const cyr_sim_desc *d, const SimGeometry &g, const void *d
   const uint32_t nvol = mbasis_volumes(g.M), pass_max = nvol < 7 ? nvol : 7;
   // laboratory frame: one quadratic form per pixel instead of every angle's projection
   const bool lab = d->reference_frame != 0;
+  const uint32_t nb = lab ? std::min(mbasis_energy_batch(g, sizeof(T)), g.nE) : 1;
   LaneBuffers &lb = wb.lane[0];
   CYR_TRY(ensure_buffer(lb.pol, sizeof(cx<T>) * V * pass_max));
-  CYR_TRY(ensure_buffer(lb.proj, sizeof(T) * P * (lab ? 4 : g.nA)));
-  CYR_TRY(ensure_buffer(lb.avg, sizeof(double) * P));
+  CYR_TRY(ensure_buffer(lb.proj, sizeof(T) * P * (lab ? 4 * (uint64_t)nb : g.nA)));
+  CYR_TRY(ensure_buffer(lb.avg, sizeof(double) * P * nb));
   CYR_TRY(ensure_buffer(lb.bins, sizeof(int32_t) * P + sizeof(uint32_t) * mbasis_bmax_words(g.nx, g.ny)));
   CYR_TRY(ensure_buffer(lb.dc, sizeof(double2) * nvol * (g.nz + 1)));
   cx<T> *pol = reinterpret_cast<cx<T> *>(lb.pol.ptr);
@@ -590,42 +606,51 @@ cyr_status run_mbasis(const cyr_sim_desc *d, const SimGeometry &g, const void *d
       }
       CYR_TRY(launch_mbasis_dcsum(dcz, nvol, g.nz, three, dcv, s));
       // ---- per energy: fields at the Ewald bins, angles, rotation
-      for (uint32_t e = e0; e < e0 + ne; e++) {
-        const double kv[3] = {g.kvec[k].x, g.kvec[k].y, g.kvec[k].z};
-        const EwaldGeom geo = make_ewald_geom(g.nx, g.ny, g.nz, d->phys_size, g.kmag[e], kv, d->interpolation);
-        MbAnglesArgs<T> aa{};
-        aa.g = geo;
-        aa.band = band;
-        aa.own = own;
-        aa.mir = mir;
-        aa.J = J;
-        aa.dcv = dcv;
-        aa.mats = su.mats + (size_t)e * g.M;
-        aa.M = g.M;
-        aa.amat = su.amat + (size_t)kEbAmat * k * g.nA;
-        aa.nA = g.nA;
-        aa.Q = m3<T>(Q);
-        if (lab) aa.quad = proj;
-        else aa.proj = proj;
-        const double *inv = su.rotinv + (size_t)6 * k * g.nA;
-        if (lab) {
+      const double kv[3] = {g.kvec[k].x, g.kvec[k].y, g.kvec[k].z};
+      const double *inv = su.rotinv + (size_t)6 * k * g.nA;
+      MbAnglesArgs<T> aa{};
+      aa.band = band;
+      aa.own = own;
+      aa.mir = mir;
+      aa.J = J;
+      aa.dcv = dcv;
+      aa.M = g.M;
+      aa.amat = su.amat + (size_t)kEbAmat * k * g.nA;
+      aa.nA = g.nA;
+      aa.Q = m3<T>(Q);
+      if (lab) {
+        // batches of nb energies: quadratic forms, rotation + average, detector warp
+        for (uint32_t b0 = e0; b0 < e0 + ne; b0 += nb) {
+          const uint32_t n = std::min(nb, e0 + ne - b0);
+          aa.g = make_ewald_geom(g.nx, g.ny, g.nz, d->phys_size, g.kmag[b0], kv, d->interpolation);
+          aa.kmag = su.kmag + b0;
+          aa.ne = n;
+          aa.mats = su.mats + (size_t)b0 * g.M;
+          aa.quad = proj;
           kt_begin(s);
           CYR_TRY(launch_mbasis_angles<T>(aa, ng, s));
-          kt_end("mbasis_quad", s, 2.0 * ng * nvol * (double)P * cxb + 4.0 * P * sizeof(T));
+          kt_end("mbasis_quad", s, n * (2.0 * ng * nvol * (double)P * cxb + 4.0 * P * sizeof(T)));
           kt_begin(s);
           CYR_TRY(launch_rotate_quad<T>(proj, inv, su.quad + (size_t)3 * k * g.nA, g.nA, g.nx, g.ny, d->rot_mask, avg,
-                                        s));
-          kt_end("rotate_quad", s, 4.0 * P * sizeof(T) + 8.0 * P);
-        } else {
+                                        n, su.tiles, s));
+          kt_end("rotate_quad", s, n * (4.0 * P * sizeof(T) + 8.0 * P));
+          CYR_TRY(launch_detector_warp_n<T>(avg, d_out + ((uint64_t)b0 * g.nK + k) * P, (uint64_t)g.nK * P, n, g.nx,
+                                            g.ny, &g.det_inv[(size_t)k * 6], d->rot_mask, s));
+        }
+      } else {
+        for (uint32_t e = e0; e < e0 + ne; e++) {
+          aa.g = make_ewald_geom(g.nx, g.ny, g.nz, d->phys_size, g.kmag[e], kv, d->interpolation);
+          aa.mats = su.mats + (size_t)e * g.M;
+          aa.proj = proj;
           kt_begin(s);
           CYR_TRY(launch_mbasis_angles<T>(aa, ng, s));
           kt_end("mbasis_angles", s, 2.0 * ng * nvol * (double)P * cxb + (double)g.nA * P * sizeof(T));
           kt_begin(s);
-          CYR_TRY(launch_rotate_all<T>(proj, inv, g.nA, g.nx, g.ny, d->rot_mask, avg, s));
+          CYR_TRY(launch_rotate_all<T>(proj, inv, g.nA, g.nx, g.ny, d->rot_mask, avg, su.tiles, s));
           kt_end("rotate_all", s, (double)(g.nA + 1) * P * sizeof(T));
+          T *out = d_out + ((uint64_t)e * g.nK + k) * P;
+          CYR_TRY(launch_detector_warp<T>(avg, out, g.nx, g.ny, &g.det_inv[(size_t)k * 6], d->rot_mask, s));
         }
-        T *out = d_out + ((uint64_t)e * g.nK + k) * P;
-        CYR_TRY(launch_detector_warp<T>(avg, out, g.nx, g.ny, &g.det_inv[(size_t)k * 6], d->rot_mask, s));
       }
       e0 += ne;
     }
@@ -787,9 +812,10 @@ uint64_t work_bytes(const cyr_sim_desc *d, const SimGeometry &g) {
     // band table, DC sums, per-angle projections, average
     const uint64_t nv = mbasis_volumes(g.M), pass = nv < 7 ? nv : 7;
     const uint64_t J = mbasis_band_cap(g) > 2 ? mbasis_band_cap(g) : 2;
-    const uint64_t planes = d->reference_frame != 0 ? 4 : g.nA;
+    const uint64_t nb = d->reference_frame != 0 ? std::min(mbasis_energy_batch(g, rs), g.nE) : 1;
+    const uint64_t planes = d->reference_frame != 0 ? 4 * nb : g.nA;
     b += cs * g.V * pass + cs * 2 * nv * J * g.P + 4 * g.P + 4 * mbasis_bmax_words(g.nx, g.ny) +
-         16 * nv * (g.nz + 1) + rs * g.P * planes + 8 * g.P;
+         16 * nv * (g.nz + 1) + rs * g.P * planes + 8 * g.P * nb;
   } else if (fused) {  // per lane: fields, bin table, Y/z partial sums (chunks x 2 bins x fields), DC sums,
                  // per-angle projections, average
     const uint64_t nch = ebasis_max_chunks(g.nx, g.ny, g.nz, d->precision);

@@ -10,7 +10,10 @@
 // [0, Nx-1] x [0, Ny-1]; taps of exactly zero weight are skipped.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <type_traits>
+#include <utility>
+#include <vector>
 
 #include "geom.hpp"
 #include "kernels.hpp"
@@ -24,11 +27,18 @@ struct Inv6 { double c[6]; };
 // Pixel of this thread in the rotation kernels: 256-thread blocks cover 16 x 16
 // tiles, each wave an 8 x 8 quadrant, so a wave's rotated source footprint is
 // compact (a 64-pixel row segment maps to a 64-pixel line of source rows at
-// most angles, touching many more cache lines).
-__device__ inline bool tile_pixel(uint32_t nx, uint32_t ny, uint32_t &X, uint32_t &Y) {
+// most angles, touching many more cache lines).  Tiles come from `tiles`
+// (ty << 16 | tx, sorted by distance from the rotation centre) in XCD order:
+// XCD x gets the x-th annulus, and a rotation about the centre maps an annulus
+// onto itself, so each XCD's L2 holds only its annulus of the source planes.
+__device__ inline bool tile_pixel(const uint32_t *__restrict__ tiles, uint32_t ntiles, uint32_t nx, uint32_t ny,
+                                  uint32_t &X, uint32_t &Y) {
+  const uint32_t q = xcd_tile(blockIdx.x, gridDim.x);
+  if (q >= ntiles) return false;
+  const uint32_t tt = tiles[q];
   const uint32_t w = threadIdx.x >> 6, l = threadIdx.x & 63;
-  X = blockIdx.x * 16 + (w & 1) * 8 + (l & 7);
-  Y = blockIdx.y * 16 + (w >> 1) * 8 + (l >> 3);
+  X = (tt & 0xffffu) * 16 + (w & 1) * 8 + (l & 7);
+  Y = (tt >> 16) * 16 + (w >> 1) * 8 + (l >> 3);
   return X < nx && Y < ny;
 }
 
@@ -66,10 +76,11 @@ __global__ void __launch_bounds__(256) average_kernel(const double *__restrict__
 template <typename T>
 __global__ void __launch_bounds__(256) rotate_all_kernel(const T *__restrict__ proj, const double *__restrict__ inv_tab,
                                                          uint32_t nA, uint32_t nx, uint32_t ny, int rot_mask,
-                                                         double *__restrict__ avg) {
+                                                         double *__restrict__ avg, const uint32_t *__restrict__ tiles,
+                                                         uint32_t ntiles) {
   const uint64_t P = (uint64_t)nx * ny;
   uint32_t X, Y;
-  if (!tile_pixel(nx, ny, X, Y)) return;
+  if (!tile_pixel(tiles, ntiles, nx, ny, X, Y)) return;
   const uint64_t t = (uint64_t)Y * nx + X;
   double acc = 0;
   uint32_t cnt = 0;
@@ -99,19 +110,31 @@ __global__ void __launch_bounds__(256) rotate_all_kernel(const T *__restrict__ p
 // q = (q0, q1, q2): I(theta) = c^2 q0 + s^2 q1 + 2 c s q2.  Bilinear sampling is
 // linear, so the three components are sampled (same taps, weights and validity
 // as bilinear_sample; a tap of zero weight contributes nothing, NaN included)
-// and combined once per angle.
-template <typename T>
+// and combined once per angle.  Each angle's back-mapped point, taps and
+// weights serve NE energies (grid y = group of NE energies, ne in total).
+template <typename T, int NE>
 __global__ void __launch_bounds__(256) rotate_quad_kernel(const T *__restrict__ quad, const double *__restrict__ inv_tab,
                                                           const double *__restrict__ coef, uint32_t nA, uint32_t nx,
-                                                          uint32_t ny, int rot_mask, double *__restrict__ avg) {
+                                                          uint32_t ny, int rot_mask, double *__restrict__ avg,
+                                                          uint32_t ne, const uint32_t *__restrict__ tiles,
+                                                          uint32_t ntiles) {
+  const uint64_t P = (uint64_t)nx * ny;
   uint32_t X, Y;
-  if (!tile_pixel(nx, ny, X, Y)) return;
+  if (!tile_pixel(tiles, ntiles, nx, ny, X, Y)) return;
   const uint64_t t = (uint64_t)Y * nx + X;
+  const uint32_t e0 = blockIdx.z * NE;
   using V4 = typename std::conditional<sizeof(T) == 4, float4, double4>::type;
-  const V4 *q4 = reinterpret_cast<const V4 *>(quad);
-  double acc = 0;
-  uint32_t cnt = 0;
-#pragma unroll 4
+  const V4 *q4[NE];
+#pragma unroll
+  for (int e = 0; e < NE; e++) q4[e] = reinterpret_cast<const V4 *>(quad) + (uint64_t)(e0 + e < ne ? e0 + e : e0) * P;
+  double acc[NE];
+  uint32_t cnt[NE];
+#pragma unroll
+  for (int e = 0; e < NE; e++) {
+    acc[e] = 0;
+    cnt[e] = 0;
+  }
+#pragma unroll 2
   for (uint32_t a = 0; a < nA; a++) {
     double sx, sy;
     back_map_point(inv_tab + 6 * (uint64_t)a, X, Y, sx, sy);
@@ -120,38 +143,50 @@ __global__ void __launch_bounds__(256) rotate_quad_kernel(const T *__restrict__ 
     const uint32_t x0 = (uint32_t)fx0, y0 = (uint32_t)fy0;
     const double fx = inside ? sx - fx0 : 0.0, fy = inside ? sy - fy0 : 0.0;
     const uint32_t x1 = x0 + 1 < nx ? x0 + 1 : x0, y1 = y0 + 1 < ny ? y0 + 1 : y0;
-    const uint32_t r0 = y0 * nx, r1 = y1 * nx;  // P < 2^32
-    const V4 a00 = q4[r0 + x0], a10 = q4[r0 + x1], a01 = q4[r1 + x0], a11 = q4[r1 + x1];
+    const uint32_t off[4] = {y0 * nx + x0, y0 * nx + x1, y1 * nx + x0, y1 * nx + x1};  // P < 2^32
     const double gx = 1 - fx, gy = 1 - fy;
     const double w[4] = {gx * gy, fx * gy, gx * fy, fx * fy};
-    const V4 tap[4] = {a00, a10, a01, a11};
-    double s0 = 0, s1 = 0, s2 = 0;
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-      const bool z = w[i] == 0;  // selected in the storage type: one move per component
-      s0 += w[i] * (double)(z ? (T)0 : tap[i].x);
-      s1 += w[i] * (double)(z ? (T)0 : tap[i].y);
-      s2 += w[i] * (double)(z ? (T)0 : tap[i].z);
-    }
     const double *c = coef + 3 * (uint64_t)a;
-    const double v = inside ? c[0] * s0 + c[1] * s1 + c[2] * s2 : __longlong_as_double(0x7ff8000000000000ll);
-    if (rot_mask) {
-      const bool ok = !isnan(v);
-      acc += ok ? v : 0.0;
-      cnt += ok ? 1u : 0u;
-    } else {
-      acc += v;
+    const double c0 = c[0], c1 = c[1], c2 = c[2];
+#pragma unroll
+    for (int e = 0; e < NE; e++) {
+      double s0 = 0, s1 = 0, s2 = 0;
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const V4 tp = q4[e][off[i]];
+        const bool z = w[i] == 0;  // selected in the storage type: one move per component
+        s0 += w[i] * (double)(z ? (T)0 : tp.x);
+        s1 += w[i] * (double)(z ? (T)0 : tp.y);
+        s2 += w[i] * (double)(z ? (T)0 : tp.z);
+      }
+      const double v = inside ? c0 * s0 + c1 * s1 + c2 * s2 : __longlong_as_double(0x7ff8000000000000ll);
+      if (rot_mask) {
+        const bool ok = !isnan(v);
+        acc[e] += ok ? v : 0.0;
+        cnt[e] += ok ? 1u : 0u;
+      } else {
+        acc[e] += v;
+      }
     }
   }
-  if (rot_mask) avg[t] = cnt == 0 ? 0.0 : acc * (1.0 / (cnt * 1.0));
-  else avg[t] = acc * (1.0 / (double)nA);
+#pragma unroll
+  for (int e = 0; e < NE; e++) {
+    if (e0 + e >= ne) break;
+    double *o = avg + (uint64_t)(e0 + e) * P;
+    if (rot_mask) o[t] = cnt[e] == 0 ? 0.0 : acc[e] * (1.0 / (cnt[e] * 1.0));
+    else o[t] = acc[e] * (1.0 / (double)nA);
+  }
 }
 
+// grid y = pattern: avg + y P -> out + y out_stride
 template <typename T>
 __global__ void __launch_bounds__(256) detector_warp_kernel(const double *__restrict__ avg, T *__restrict__ out,
-                                                            uint32_t nx, uint32_t ny, Inv6 inv, int rot_mask) {
+                                                            uint32_t nx, uint32_t ny, Inv6 inv, int rot_mask,
+                                                            uint64_t out_stride) {
   const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (uint64_t)nx * ny) return;
+  avg += (uint64_t)blockIdx.y * nx * ny;
+  out += (uint64_t)blockIdx.y * out_stride;
   const uint32_t Y = (uint32_t)(t / nx), X = (uint32_t)(t - (uint64_t)Y * nx);
   double v;
   if (!bilinear_sample<double>(avg, nx, ny, inv.c, X, Y, &v))
@@ -177,6 +212,21 @@ Inv6 mk_inv(const double inv[6]) {
 
 }  // namespace
 
+uint32_t rotation_tile_count(uint32_t nx, uint32_t ny) { return ceil_div(nx, 16) * ceil_div(ny, 16); }
+
+void rotation_tile_order(uint32_t nx, uint32_t ny, uint32_t *out) {
+  const uint32_t tx = ceil_div(nx, 16), ty = ceil_div(ny, 16);
+  std::vector<std::pair<double, uint32_t>> r;
+  r.reserve((size_t)tx * ty);
+  for (uint32_t y = 0; y < ty; y++)
+    for (uint32_t x = 0; x < tx; x++) {
+      const double dx = (x * 16.0 + 8.0) - nx / 2.0, dy = (y * 16.0 + 8.0) - ny / 2.0;
+      r.emplace_back(dx * dx + dy * dy, (y << 16) | x);
+    }
+  std::stable_sort(r.begin(), r.end(), [](const auto &a, const auto &b) { return a.first < b.first; });
+  for (size_t i = 0; i < r.size(); i++) out[i] = r[i].second;
+}
+
 template <typename T>
 cyr_status launch_rotate_accumulate(const T *proj, double *acc, uint32_t *count, uint32_t nx, uint32_t ny,
                                     const double inv[6], int rot_mask, hipStream_t st) {
@@ -196,22 +246,26 @@ cyr_status launch_average(const double *acc, const uint32_t *count, double *avg,
 
 template <typename T>
 cyr_status launch_rotate_all(const T *proj, const double *inv_tab, uint32_t nA, uint32_t nx, uint32_t ny, int rot_mask,
-                             double *avg, hipStream_t st) {
-  const uint64_t P = (uint64_t)nx * ny;
-  (void)P;
-  hipLaunchKernelGGL((rotate_all_kernel<T>), dim3(ceil_div(nx, 16), ceil_div(ny, 16)), dim3(256), 0, st, proj,
-                     inv_tab, nA, nx, ny, rot_mask, avg);
+                             double *avg, const uint32_t *tiles, hipStream_t st) {
+  const uint32_t ntiles = rotation_tile_count(nx, ny);
+  if (!tiles) return fail(CYR_E_INVALID, "rotation: tile order missing");
+  hipLaunchKernelGGL((rotate_all_kernel<T>), dim3(xcd_grid(ntiles)), dim3(256), 0, st, proj, inv_tab, nA, nx, ny,
+                     rot_mask, avg, tiles, ntiles);
   CYR_LAUNCHED();
   return CYR_OK;
 }
 
 template <typename T>
 cyr_status launch_rotate_quad(const T *quad, const double *inv_tab, const double *coef, uint32_t nA, uint32_t nx,
-                              uint32_t ny, int rot_mask, double *avg, hipStream_t st) {
-  const uint64_t P = (uint64_t)nx * ny;
-  (void)P;
-  hipLaunchKernelGGL((rotate_quad_kernel<T>), dim3(ceil_div(nx, 16), ceil_div(ny, 16)), dim3(256), 0, st, quad,
-                     inv_tab, coef, nA, nx, ny, rot_mask, avg);
+                              uint32_t ny, int rot_mask, double *avg, uint32_t ne, const uint32_t *tiles,
+                              hipStream_t st) {
+  constexpr int NE = 4;
+  if (ne == 0) return CYR_OK;
+  if ((uint64_t)nx * ny > 0xffffffffull) return fail(CYR_E_UNSUPPORTED, "rotation: more than 2^32 pixels");
+  if (!tiles) return fail(CYR_E_INVALID, "rotation: tile order missing");
+  const uint32_t ntiles = rotation_tile_count(nx, ny);
+  hipLaunchKernelGGL((rotate_quad_kernel<T, NE>), dim3(xcd_grid(ntiles), 1, ceil_div(ne, NE)), dim3(256), 0, st,
+                     quad, inv_tab, coef, nA, nx, ny, rot_mask, avg, ne, tiles, ntiles);
   CYR_LAUNCHED();
   return CYR_OK;
 }
@@ -221,7 +275,18 @@ cyr_status launch_detector_warp(const double *avg, T *out, uint32_t nx, uint32_t
                                 hipStream_t st) {
   const uint64_t P = (uint64_t)nx * ny;
   hipLaunchKernelGGL((detector_warp_kernel<T>), dim3(ceil_div(P, 256)), dim3(256), 0, st, avg, out, nx, ny,
-                     mk_inv(inv), rot_mask);
+                     mk_inv(inv), rot_mask, (uint64_t)0);
+  CYR_LAUNCHED();
+  return CYR_OK;
+}
+
+template <typename T>
+cyr_status launch_detector_warp_n(const double *avg, T *out, uint64_t out_stride, uint32_t ne, uint32_t nx,
+                                  uint32_t ny, const double inv[6], int rot_mask, hipStream_t st) {
+  const uint64_t P = (uint64_t)nx * ny;
+  if (ne == 0) return CYR_OK;
+  hipLaunchKernelGGL((detector_warp_kernel<T>), dim3(ceil_div(P, 256), ne), dim3(256), 0, st, avg, out, nx, ny,
+                     mk_inv(inv), rot_mask, out_stride);
   CYR_LAUNCHED();
   return CYR_OK;
 }
@@ -238,9 +303,11 @@ cyr_status launch_warp(const T *src, T *dst, uint32_t nx, uint32_t ny, const dou
   template cyr_status launch_rotate_accumulate<T>(const T *, double *, uint32_t *, uint32_t, uint32_t,             \
                                                   const double *, int, hipStream_t);                               \
   template cyr_status launch_rotate_all<T>(const T *, const double *, uint32_t, uint32_t, uint32_t, int, double *, \
-                                           hipStream_t);                                                           \
+                                           const uint32_t *, hipStream_t);                                         \
   template cyr_status launch_rotate_quad<T>(const T *, const double *, const double *, uint32_t, uint32_t,        \
-                                            uint32_t, int, double *, hipStream_t);                                 \
+                                            uint32_t, int, double *, uint32_t, const uint32_t *, hipStream_t);     \
+  template cyr_status launch_detector_warp_n<T>(const double *, T *, uint64_t, uint32_t, uint32_t, uint32_t,       \
+                                                const double *, int, hipStream_t);                                 \
   template cyr_status launch_detector_warp<T>(const double *, T *, uint32_t, uint32_t, const double *, int,        \
                                               hipStream_t);                                                        \
   template cyr_status launch_warp<T>(const T *, T *, uint32_t, uint32_t, const double *, hipStream_t);
