@@ -30,9 +30,10 @@ import time
 METRIC = "voxel·angle·energy/s + p50/p99 latency, batched vs plain, at ≤1e-4 rel. L2"
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this
-# workload, scaled by the counters' reading for a copy kernel of known size
-# (profiles/ holds the runs); None where not measured.
-TRAFFIC = {}
+# workload (material-basis executor), scaled by the counters' reading for a 1 GiB
+# negation (fetch x2.0, write x1.0); absent where not measured.
+TRAFFIC = {"mbasis": {"xfields": 2.954e9, "fft_y": 3.758e9, "mbasis_zband": 2.017e9, "mbasis_quad": 3.338e9,
+                      "rotate_quad": 1.248e9}}
 # timing labels of the library (cyr_kernel_timing) -> kernel symbol
 KERNEL_OF = {"xfields": "xfields_kernel", "fft_y": "fft_ct_kernel (Y pass)", "fft_z": "fft_ct_kernel (Z pass)",
              "mbasis_extract": "mbasis_extract_kernel", "mbasis_zband": "mbasis_zband_kernel", "mbasis_angles": "mbasis_angles_kernel",
@@ -250,7 +251,7 @@ def main(argv=None):
     dom = max(kern, key=lambda n: kern[n]["total_us"])
     ach = table[dom]["GBs"]
     result["roofline"] = {"bound": "hbm", "kernel": KERNEL_OF.get(dom, dom), "achieved": ach, "peak": HBM_PEAK_GBS,
-                          "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": TRAFFIC.get(dom),
+                          "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": TRAFFIC.get(a.pipeline, {}).get(dom),
                           "algorithmic_bytes": kern[dom]["avg_bytes"], "avg_us": kern[dom]["avg_us"],
                           "kernels": table}
 

@@ -6,14 +6,17 @@ S, sphere-uniform Euler angles), 8-32 energies over 280-300 eV and 18 E angles
 (0:10:170), linear Ewald interpolation, no window, FP32.  Throughput is
 voxel*angle*energy per second over the whole batch (inputs resident in HBM).
 
-Modes, all on the same device buffers:
+Modes, all on the same device buffers (P = the executor under test, --pipeline,
+default the material-basis executor):
   plain_serial    the reference's per-angle structure, one simulation at a time
-  ebasis_serial   the E-basis executor, one simulation at a time
-  ebasis_batch    cyr_simulate_batch (members over 4 streams, shapes autotuned)
-  ebasis_graph    a cyr_batch plan (the same schedule captured in a hipGraph),
+  ebasis_serial   the E-basis executor, one simulation at a time (its outputs are
+                  the comparison reference of the other modes)
+  P_serial        executor P, one simulation at a time
+  P_batch         cyr_simulate_batch (members over 4 streams, shapes autotuned)
+  P_graph         a cyr_batch plan (the same schedule captured in a hipGraph),
                   replayed
 
-    python -m cyrsoxs_amd.bench_batch [--count 64] [--plain-count N] [--reps 2]
+    python -m cyrsoxs_amd.bench_batch [--count 64] [--plain-count N] [--reps 2] [--pipeline mbasis|ebasis]
 
 One JSON line on stdout.  --plain-count times the plain path on the first N
 members (in the batch's own order) and scales by their share of the units when
@@ -33,6 +36,7 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--reps", type=int, default=2)
     ap.add_argument("--plain-count", type=int, default=16)
+    ap.add_argument("--pipeline", choices=("mbasis", "ebasis"), default="mbasis")
     a = ap.parse_args(argv)
 
     import numpy as np
@@ -41,6 +45,8 @@ def main(argv=None):
     from cyrsoxs_amd import capi, synthetic
 
     torch.cuda.set_device(0)
+    pn = a.pipeline
+    PIPE = capi.PIPE_MBASIS if pn == "mbasis" else capi.PIPE_EBASIS
     members = synthetic.config3_members(a.count, a.seed)
     sims, vox, mats, outs = [], [], [], []
     units = []
@@ -52,7 +58,7 @@ def main(argv=None):
         outs.append(torch.empty((len(m["energies"]), 1, ny, nx), dtype=torch.float32, device="cuda"))
         kw = dict(interpolation=m["interpolation"], windowing=m["windowing"], precision=capi.FP32)
         sims.append({p: capi.Sim(m["dims"], M, m["phys_size"], m["energies"], m["angles"], pipeline=p, **kw)
-                     for p in (capi.PIPE_PLAIN, capi.PIPE_EBASIS)})
+                     for p in (capi.PIPE_PLAIN, capi.PIPE_EBASIS, capi.PIPE_MBASIS)})
         units.append(nx * ny * nz * sims[-1][capi.PIPE_EBASIS].num_angles * len(m["energies"]))
     torch.cuda.synchronize()
     total_units = float(sum(units))
@@ -94,19 +100,28 @@ def main(argv=None):
                     worst = max(worst, float(np.linalg.norm(g[e][f].astype(np.float64) - r[e][f]) / den))
         return worst, same
 
-    capi.simulate_batch(eb, vox, mats, outs)
-    t = min(timed(lambda: capi.simulate_batch(eb, vox, mats, outs)) for _ in range(a.reps))
+    ps = [s[PIPE] for s in sims]
+    if pn != "ebasis":
+        serial(PIPE, range(a.count))
+        t = min(timed(lambda: serial(PIPE, range(a.count))) for _ in range(a.reps))
+        w, same = compare()
+        res[f"{pn}_serial"] = {"seconds": t, "value": total_units / t, "max_rel_l2_vs_ebasis": w,
+                               "nan_mask_identical": same}
+
+    capi.simulate_batch(ps, vox, mats, outs)
+    t = min(timed(lambda: capi.simulate_batch(ps, vox, mats, outs)) for _ in range(a.reps))
     w, same = compare()
-    res["ebasis_batch"] = {"seconds": t, "value": total_units / t, "max_rel_l2_vs_serial": w, "nan_mask_identical": same}
+    res[f"{pn}_batch"] = {"seconds": t, "value": total_units / t, "max_rel_l2_vs_ebasis": w,
+                          "nan_mask_identical": same}
 
     t0 = time.perf_counter()
-    plan = capi.BatchPlan(eb, vox, mats, outs)
+    plan = capi.BatchPlan(ps, vox, mats, outs)
     t_plan = time.perf_counter() - t0
     plan.launch()
     t = min(timed(plan.launch) for _ in range(a.reps))
     w, same = compare()
-    res["ebasis_graph"] = {"seconds": t, "value": total_units / t, "plan_create_s": t_plan,
-                           "max_rel_l2_vs_serial": w, "nan_mask_identical": same}
+    res[f"{pn}_graph"] = {"seconds": t, "value": total_units / t, "plan_create_s": t_plan,
+                          "max_rel_l2_vs_ebasis": w, "nan_mask_identical": same}
     plan.close()
 
     # plain one at a time on the first members, scaled by their unit share
@@ -132,10 +147,11 @@ def main(argv=None):
         "workload": f"{a.count} uneven morphologies {{128^2x32, 256^2x64, 384^2x48, 512^2x64}}, 2-4 materials, "
                     "8-32 energies, 18 angles, linear, no window, FP32",
         "total_units": total_units, "modes": res,
-        "speedup_batch_vs_plain_serial": res["ebasis_batch"]["value"] / plain_v,
-        "speedup_graph_vs_plain_serial": res["ebasis_graph"]["value"] / plain_v,
-        "speedup_batch_vs_ebasis_serial": res["ebasis_batch"]["value"] / res["ebasis_serial"]["value"],
-        "speedup_graph_vs_ebasis_serial": res["ebasis_graph"]["value"] / res["ebasis_serial"]["value"],
+        "pipeline": pn,
+        "speedup_batch_vs_plain_serial": res[f"{pn}_batch"]["value"] / plain_v,
+        "speedup_graph_vs_plain_serial": res[f"{pn}_graph"]["value"] / plain_v,
+        "speedup_batch_vs_serial": res[f"{pn}_batch"]["value"] / res[f"{pn}_serial"]["value"],
+        "speedup_graph_vs_serial": res[f"{pn}_graph"]["value"] / res[f"{pn}_serial"]["value"],
     }
     print(json.dumps(out), flush=True)
     return 0

@@ -14,10 +14,11 @@ censored, so p99 under overload is a lower bound).  Latency is submit-to-result
 on the host, measured by the engine.  Reported per rate: p50/p99 latency,
 completed requests, throughput in voxel*angle*energy/s.  The same stream is
 also served FIFO by the plain per-angle executor on one slot at the lowest
-rate (--plain-window), and by the E-basis executor on one slot (FIFO, no
-overlap) at one rate, for comparison.
+rate (--plain-window), and by the served executor (--pipeline, default the
+material-basis executor) on one slot (FIFO, no overlap) at one rate, for
+comparison.
 
-    python -m cyrsoxs_amd.bench_serving [--rates 2,5,10,20,40] [--window 15] [--slots 3]
+    python -m cyrsoxs_amd.bench_serving [--rates 2,5,10,20,40] [--window 15] [--slots 3] [--pipeline mbasis|ebasis]
 """
 from __future__ import annotations
 
@@ -48,6 +49,7 @@ def main(argv=None):
     ap.add_argument("--plain-window", type=float, default=10.0)
     ap.add_argument("--fifo-rate", type=float, default=5.0)
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--pipeline", choices=("mbasis", "ebasis"), default="mbasis")
     a = ap.parse_args(argv)
 
     import numpy as np
@@ -56,6 +58,7 @@ def main(argv=None):
     from cyrsoxs_amd import capi, synthetic
 
     torch.cuda.set_device(0)
+    PIPE = capi.PIPE_MBASIS if a.pipeline == "mbasis" else capi.PIPE_EBASIS
     grids = [tuple(int(x) for x in g.split("x")) for g in a.grids.split(",")]
     rates = [float(r) for r in a.rates.split(",")]
 
@@ -150,20 +153,21 @@ def main(argv=None):
 
     results = []
     for i, r in enumerate(rates):
-        res = serve(r, a.window, a.slots, capi.PIPE_EBASIS, a.seed * 1000 + i)
+        res = serve(r, a.window, a.slots, PIPE, a.seed * 1000 + i)
         print(json.dumps(res), file=sys.stderr, flush=True)
         results.append(res)
     # the same request stream as the continuous-batching run at that rate (or a
     # fresh one if that rate was not run), served FIFO on one slot
     fi = rates.index(a.fifo_rate) if a.fifo_rate in rates else 77
-    fifo = serve(a.fifo_rate, a.window, 1, capi.PIPE_EBASIS, a.seed * 1000 + fi)
+    fifo = serve(a.fifo_rate, a.window, 1, PIPE, a.seed * 1000 + fi)
     print(json.dumps(fifo), file=sys.stderr, flush=True)
     plain = serve(min(rates), a.plain_window, 1, capi.PIPE_PLAIN, a.seed * 1000 + rates.index(min(rates)))
     print(json.dumps(plain), file=sys.stderr, flush=True)
     out = {"metric": "p50/p99 latency and voxel·angle·energy/s under a Poisson request stream",
            "workload": "grids " + a.grids + ", 2-4 materials, 1-50 energies 280-305 eV, EAngle 0:5:175, linear, "
                        "Hanning, FP32",
-           "continuous_batching": results, "ebasis_fifo_one_slot": fifo, "plain_fifo_one_slot": plain}
+           "pipeline": a.pipeline, "continuous_batching": results, f"{a.pipeline}_fifo_one_slot": fifo,
+           "plain_fifo_one_slot": plain}
     print(json.dumps(out), flush=True)
     return 0
 

@@ -6,15 +6,16 @@ its spectral peak at q = 2 pi / 40 nm^-1, thresholded into 3 phases with a
 boundaries, S = 0.8 at interfaces and 0.2 in the bulk.  50 energies 280.0-304.5
 eV in 0.5 eV steps, EAngleRotation 0:2:358 (180 angles), 2D path.
 
-Runs, all E-basis on the GPU: FP64, FP32 (FP64 twiddle tables, FP64 sums over
-angles), FP32 with FP16 morphology storage, FP32 with BF16 storage.  Every FP32
+Runs, all with the fused executor chosen by --pipeline (default the
+material-basis executor) on the GPU: FP64, FP32 (FP64 twiddle tables, FP64 sums
+over angles), FP32 with FP16 morphology storage, FP32 with BF16 storage.  Every FP32
 run is compared per pattern against the GPU FP64 run of the same workload (the
 GPU FP64 path is held to 1e-10 relative L2 of the FP64 CPU oracle by the test
 suite, including a thin-film case).  Per pattern: relative L2 over the detector
 mask (pixels finite in the reference) and the largest pixel error relative to
 the pattern's largest value.  Also reported: throughput of each run.
 
-    python -m cyrsoxs_amd.bench_thinfilm [--n 2048] [--energies 50]
+    python -m cyrsoxs_amd.bench_thinfilm [--n 2048] [--energies 50] [--pipeline mbasis|ebasis]
 """
 from __future__ import annotations
 
@@ -29,6 +30,7 @@ def main(argv=None):
     ap.add_argument("--n", type=int, default=2048)
     ap.add_argument("--energies", type=int, default=50)
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--pipeline", choices=("mbasis", "ebasis"), default="mbasis")
     a = ap.parse_args(argv)
 
     import numpy as np
@@ -47,7 +49,7 @@ def main(argv=None):
 
     def run(precision, storage, vox):
         sim = capi.Sim((n, n, 1), 3, 1.0, energies, angles, interpolation=1, precision=precision, storage=storage,
-                       pipeline=capi.PIPE_EBASIS)
+                       pipeline=capi.PIPE_MBASIS if a.pipeline == "mbasis" else capi.PIPE_EBASIS)
         dt = torch.float64 if precision == capi.FP64 else torch.float32
         out = torch.empty((len(energies), 1, n, n), dtype=dt, device="cuda")
         dv = vox
@@ -86,7 +88,8 @@ def main(argv=None):
         print(f"{name}: {res[name]}", file=sys.stderr, flush=True)
 
     out = {"workload": f"thin film {n}x{n}x1, 3 materials, PhysSize 1 nm, {len(energies)} energies 280-"
-                       f"{energies[-1]} eV, 180 angles (0:2:358), E-basis", "units": units, "results": res}
+                       f"{energies[-1]} eV, 180 angles (0:2:358)", "pipeline": a.pipeline, "units": units,
+           "results": res}
     print(json.dumps(out), flush=True)
     return 0
 

@@ -22,7 +22,9 @@ namespace cyr {
 
 template <typename T, int E> struct RtFft {
   static constexpr int kE = E;
-  StagePlan P;
+  // the kernel argument itself (constant memory): a copy indexed by run-time
+  // stage numbers would be placed in scratch
+  const StagePlan &P;
   uint32_t tpp;
   LdsMap L;
   __device__ RtFft(const StagePlan &plan, uint32_t tpp_, uint32_t pitch, bool contig)

@@ -154,9 +154,13 @@ __global__ void __launch_bounds__(WG) mbasis_zband_kernel(BandArgs<T> a, const c
   }
 }
 
+// Columns per tile: 32 for short pencils (256 B contiguous per z row measured
+// faster than 16 or 64 at Nz = 128), else as many as 1024 threads allow.
+constexpr int zband_pp(int N) { return N <= 128 ? 32 : (N <= 512 ? 16 : (N == 1024 ? 8 : 4)); }
+
 template <typename T, int N>
 hipError_t launch_zband_n(const BandArgs<T> &a, hipStream_t st, const cx<T> *tw) {
-  constexpr int PP = N <= 512 ? 16 : (N == 1024 ? 8 : 4);
+  constexpr int PP = zband_pp(N);
   using F = typename CtPlan<T, N, false, PP>::type;
   constexpr int WG = PP * F::TPP;
   const uint64_t tiles = (uint64_t)((a.nx + PP - 1) / PP) * a.ny * a.count;
@@ -442,7 +446,7 @@ template <typename T> cyr_status launch_mbasis_zband(const BandArgs<T> &a, hipSt
   const int prec = sizeof(T) == 8 ? CYR_FP64 : CYR_FP32;
   if (a.count == 0 || a.nz < 2 || !ct_usable(a.nz, prec)) return CYR_OK;
   if (a.count > (uint32_t)kXMaxOut || a.J == 0) return fail(CYR_E_INVALID, "Z band pass: bad shape");
-  const uint32_t pp = a.nz <= 512 ? 16 : (a.nz == 1024 ? 8 : 4);
+  const uint32_t pp = a.nz <= 128 ? 32 : (a.nz <= 512 ? 16 : (a.nz == 1024 ? 8 : 4));
   if (((size_t)pp * a.nz + a.nz) * sizeof(cx<T>) > device_max_lds()) return CYR_OK;
   cyr_status s;
   const cx<T> *tw = twiddles<T>(a.nz, &s);

@@ -439,10 +439,10 @@ std::mutex g_band_mu;
 std::map<std::vector<double>, uint32_t> g_band_widths;
 
 // Band table of energies [e0, e0 + ne) for k vector k into `band`, and its widest
-// band in *J (read back once per distinct geometry; resident runs need it cached).
+// band in *J (read back once per distinct geometry, which waits for the stream;
+// a stream under graph capture needs it cached by an earlier run).
 cyr_status mbasis_band(const cyr_sim_desc *d, const SimGeometry &g, uint32_t k, uint32_t e0, uint32_t ne,
-                       const double *d_kmag, int32_t *band, uint32_t *bmax, hipStream_t s, RunMode mode,
-                       uint32_t *J) {
+                       const double *d_kmag, int32_t *band, uint32_t *bmax, hipStream_t s, uint32_t *J) {
   const double kv[3] = {g.kvec[k].x, g.kvec[k].y, g.kvec[k].z};
   const EwaldGeom geo = make_ewald_geom(g.nx, g.ny, g.nz, d->phys_size, g.kmag[e0], kv, d->interpolation);
   CYR_TRY(launch_mbasis_bands(geo, d_kmag + e0, ne, band, bmax, s));
@@ -457,7 +457,9 @@ cyr_status mbasis_band(const cyr_sim_desc *d, const SimGeometry &g, uint32_t k, 
       return CYR_OK;
     }
   }
-  if (mode == kRunResident)
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  CYR_HIP(hipStreamIsCapturing(s, &cap));
+  if (cap != hipStreamCaptureStatusNone)
     return fail(CYR_E_INVALID, "material basis: Ewald band widths unknown under capture (run the simulation once first)");
   uint32_t h = 0;
   CYR_HIP(hipMemcpyAsync(&h, bmax, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
@@ -493,7 +495,7 @@ uint32_t mbasis_band_cap(const SimGeometry &g) {
 // every angle as 2D work.
 template <typename T>
 cyr_status run_mbasis(const cyr_sim_desc *d, const SimGeometry &g, const void *d_vox, T *d_out, WorkBuffers &wb,
-                      hipStream_t s, const Setup<T> &su, RunMode mode) {
+                      hipStream_t s, const Setup<T> &su) {
   const uint64_t V = g.V, P = g.P;
   const bool hann = d->windowing == 1 && d->algorithm != 1;
   const bool three = g.nz > 1;
@@ -525,7 +527,7 @@ cyr_status run_mbasis(const cyr_sim_desc *d, const SimGeometry &g, const void *d
       uint32_t ne = g.nE - e0, J = 0;
       for (;;) {
         kt_begin(s);
-        CYR_TRY(mbasis_band(d, g, k, e0, ne, su.kmag, band, bmax, s, mode, &J));
+        CYR_TRY(mbasis_band(d, g, k, e0, ne, su.kmag, band, bmax, s, &J));
         kt_end("mbasis_bands", s, (double)P * 4);
         if (J <= cap || ne == 1) break;
         ne = (ne + 1) / 2;
@@ -675,7 +677,7 @@ cyr_status run_impl(const cyr_sim_desc *d, const SimGeometry &g, const void *d_v
   CYR_TRY(setup_views<T>(d, g, tb, tables, su));
   const bool fused = d->pipeline != CYR_PIPE_PLAIN && ebasis_usable(d, g);
   const cyr_status s = !fused ? run_plain<T>(d, g, d_vox, d_out, wb, st, su)
-                       : d->pipeline == CYR_PIPE_MBASIS ? run_mbasis<T>(d, g, d_vox, d_out, wb, st, su, mode)
+                       : d->pipeline == CYR_PIPE_MBASIS ? run_mbasis<T>(d, g, d_vox, d_out, wb, st, su)
                                                        : run_ebasis<T>(d, g, d_vox, d_out, wb, st, su);
   if (mode != kRunSync) return s;
   // drain the stream on errors too, so no queued launch outlives the call's buffers

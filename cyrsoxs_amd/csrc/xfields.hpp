@@ -26,6 +26,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "fft.hpp"
 #include "fftpolicy.hpp"
 #include "kernels.hpp"
@@ -35,6 +37,16 @@
 namespace cyr {
 
 enum XMode { kXTensor = 0, kXPair = 1, kXSingle = 2 };
+
+// Compile-time loop: fn(std::integral_constant<int, I>) for I in [I0, N).  Used
+// where a #pragma unroll may be declined (large bodies), which would turn
+// register arrays indexed by the loop variable into scratch memory.
+template <int I, int N, class Fn> __device__ inline void static_for(Fn &&fn) {
+  if constexpr (I < N) {
+    fn(std::integral_constant<int, I>{});
+    static_for<I + 1, N>(fn);
+  }
+}
 template <int MODE> constexpr int x_outputs() { return MODE == kXTensor ? kEbFields : (MODE == kXPair ? 7 : 4); }
 
 // ---- the five tensor fields of one voxel -------------------------------------
@@ -203,8 +215,8 @@ __global__ void __launch_bounds__(WG, (MODE == kXTensor && F::kE <= 8) ? 4 : 2)
       fim[c][i] = fi[c];
     }
   }
-#pragma unroll
-  for (int c = 0; c < NO; c++) {
+  static_for<0, NO>([&](auto cc) {
+    constexpr int c = decltype(cc)::value;
     cx<T> v[E];
 #pragma unroll
     for (int i = 0; i < E; i++) v[i] = mk<T>(fre[c][i], fim[c][i]);
@@ -216,7 +228,7 @@ __global__ void __launch_bounds__(WG, (MODE == kXTensor && F::kE <= 8) ? 4 : 2)
     }
     // (run() ends with a barrier after its last LDS reads, so the next field's
     // first LDS store cannot overtake them)
-  }
+  });
 }
 
 // ---- plan helpers shared by the fused executors ------------------------------

@@ -17,9 +17,10 @@ torch = pytest.importorskip("torch")
 MEMBERS = [
     dict(dims=(32, 32, 16), M=2, morph=0, pipeline=capi.PIPE_EBASIS, kw=dict(interpolation=1, windowing=1)),
     dict(dims=(48, 48, 24), M=3, morph=0, pipeline=capi.PIPE_EBASIS, kw=dict(interpolation=0)),
-    dict(dims=(64, 64, 1), M=2, morph=1, pipeline=capi.PIPE_EBASIS, kw=dict(interpolation=1)),
+    dict(dims=(64, 64, 1), M=2, morph=1, pipeline=capi.PIPE_MBASIS, kw=dict(interpolation=1)),
     dict(dims=(24, 24, 12), M=2, morph=0, pipeline=capi.PIPE_PLAIN, kw=dict(interpolation=1, rot_mask=1)),
     dict(dims=(64, 64, 32), M=4, morph=0, pipeline=capi.PIPE_EBASIS, kw=dict(interpolation=1)),
+    dict(dims=(40, 40, 64), M=3, morph=1, pipeline=capi.PIPE_MBASIS, kw=dict(interpolation=1, windowing=1)),
 ]
 
 
@@ -68,7 +69,7 @@ def test_batch_member_matches_oracle():
 def test_batch_plan_graph_replays():
     """Every replay of the captured graph equals the eager result; a replay after
     the morphology changed in place equals a fresh run on the new data."""
-    members = [_member(i, c) for i, c in enumerate(MEMBERS) if i in (0, 2, 4)]
+    members = [_member(i, c) for i, c in enumerate(MEMBERS) if i in (0, 2, 4, 5)]
     d_vox = [torch.from_numpy(v).cuda() for _, v, _ in members]
     d_out = [torch.empty((len(s.energies), 1, s.dims[1], s.dims[0]), dtype=torch.float32, device="cuda")
              for s, _, _ in members]
