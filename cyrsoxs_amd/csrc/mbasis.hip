@@ -352,16 +352,20 @@ __global__ void __launch_bounds__(256) mbasis_angles_kernel(MbAnglesArgs<T> a) {
 template <typename T, int NG>
 __global__ void __launch_bounds__(256) mbasis_quad_kernel(MbAnglesArgs<T> a) {
   const uint64_t P2 = (uint64_t)a.g.nx * a.g.ny;
-  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  // 1D grid in XCD order over (pixel block, energy), energy fastest: the ne
+  // energies of a pixel block read the same band columns and run on one XCD
+  // back to back, so all but the first find them in its L2
+  const uint32_t q = xcd_tile(blockIdx.x, gridDim.x);
+  const uint32_t e = q % a.ne, b = q / a.ne;
+  const uint64_t t = (uint64_t)b * blockDim.x + threadIdx.x;
   if (t >= P2) return;
-  // grid y = energy of the group: its |k|, constants and output plane
-  a.g.k = a.kmag[blockIdx.y];
-  a.mats += (uint64_t)blockIdx.y * a.M;
+  a.g.k = a.kmag[e];
+  a.mats += (uint64_t)e * a.M;
   const uint32_t Y = (uint32_t)(t / a.g.nx), X = (uint32_t)(t - (uint64_t)Y * a.g.nx);
   cx<T> F[NG][kEbFields];
   double md[NG][6];
   T w1;
-  T *o = a.quad + ((uint64_t)blockIdx.y * P2 + t) * 4;
+  T *o = a.quad + ((uint64_t)e * P2 + t) * 4;
   if (!mb_fields<T, NG>(a, X, Y, F, md, w1)) {
     o[0] = o[1] = o[2] = mb_qnan<T>();
     o[3] = 0;
@@ -474,7 +478,9 @@ template <typename T> cyr_status launch_mbasis_angles(const MbAnglesArgs<T> &a, 
   const dim3 grid(ceil_div(P, 256)), block(256);
   if (a.quad) {
     if (!a.kmag || a.ne == 0 || a.ne > 65535) return fail(CYR_E_INVALID, "quadratic form: energy table missing");
-    const dim3 g2(ceil_div(P, 256), a.ne);
+    const uint64_t tiles = (uint64_t)ceil_div(P, 256) * a.ne;
+    if (tiles > 0x7ffffff0ull) return fail(CYR_E_UNSUPPORTED, "quadratic form: grid too large");
+    const dim3 g2(xcd_grid(tiles));
     if (ng == 2) hipLaunchKernelGGL((mbasis_quad_kernel<T, 2>), g2, block, 0, st, a);
     else hipLaunchKernelGGL((mbasis_quad_kernel<T, 1>), g2, block, 0, st, a);
   } else {
