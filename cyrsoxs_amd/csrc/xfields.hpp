@@ -2,10 +2,12 @@
 //
 // A block holds pp rows of tpp threads (F::kE elements each); WG = 64 makes
 // every row exchange wave-local (the barriers compile to LDS waits).  LDS: the
-// pp x row tile.  The X twiddles are read from the global table: with many
-// independent single-wave blocks per CU their latency hides, while staging them
-// per block cost more than it saved (as did looping a block over several rows,
-// which spills registers).
+// pp x row tile, plus (basis modes) the X twiddle table.  The tensor mode reads
+// its twiddles from the global table (5 FFTs per row: staging them per block cost
+// more than it saved, as did looping a block over several rows, which spills
+// registers); the basis modes' 4-7 FFTs per row read them from LDS (the pair
+// kernel's vector-memory instructions drop from 124 to ~34 per wave: 731 ->
+// 621 us at 512^2 x 128).
 //
 // What a voxel contributes is selected by MODE:
 //   kXTensor  the five E-basis tensor fields N'00, N'01, N'02, N'11, N'12 of one
@@ -37,6 +39,9 @@
 namespace cyr {
 
 enum XMode { kXTensor = 0, kXPair = 1, kXSingle = 2 };
+// Basis modes run 4-7 FFTs per row: their X twiddles are staged in LDS once per
+// block (the tensor mode's 5 FFTs measured faster reading the global table).
+template <int MODE> constexpr bool x_lds_twiddles() { return MODE != kXTensor; }
 
 // Compile-time loop: fn(std::integral_constant<int, I>) for I in [I0, N).  Used
 // where a #pragma unroll may be declined (large bodies), which would turn
@@ -190,6 +195,13 @@ __global__ void __launch_bounds__(WG, (MODE == kXTensor && F::kE <= 8) ? 4 : 2)
   const T k4 = (T)(1.0 / (4.0 * M_PI));
   const T wyz = hann ? a.hy[y] * a.hz[z] * k4 : k4;
   const bool rot = a.rot != 0;
+  const cx<T> *twp = a.twx;
+  if constexpr (x_lds_twiddles<MODE>()) {
+    cx<T> *ltw = lds + (size_t)a.pp * a.pitch;
+    for (uint32_t i = threadIdx.x; i < a.nx; i += WG) ltw[i] = a.twx[i];
+    lds_sync();  // (single-wave blocks: an LDS wait only)
+    twp = ltw;
+  }
   T fre[NO][E], fim[NO][E];
 #pragma unroll
   for (int i = 0; i < E; i++) {
@@ -220,7 +232,7 @@ __global__ void __launch_bounds__(WG, (MODE == kXTensor && F::kE <= 8) ? 4 : 2)
     cx<T> v[E];
 #pragma unroll
     for (int i = 0; i < E; i++) v[i] = mk<T>(fre[c][i], fim[c][i]);
-    fft.run(v, t, p, lds, a.twx);
+    fft.run(v, t, p, lds, twp);
     if (active) {
       cx<T> *o = a.out[c] + (uint64_t)row * a.nx;
 #pragma unroll
@@ -342,7 +354,7 @@ cyr_status launch_xfields(PolxArgs<T> a, const void *vox, int storage, int morph
   a.pp = wg / a.tpp;
   if (a.pp < 1) a.pp = 1;
   a.pitch = ct ? ct_row_pitch(a.nx) : a.nx;
-  const size_t lds = (size_t)a.pp * a.pitch * sizeof(cx<T>);
+  const size_t lds = ((size_t)a.pp * a.pitch + (x_lds_twiddles<MODE>() ? a.nx : 0)) * sizeof(cx<T>);
   if (lds > device_max_lds()) return fail(CYR_E_UNSUPPORTED, "fused X kernel: row does not fit in LDS");
   cyr_status s;
   a.twx = twiddles<T>(a.nx, &s);
