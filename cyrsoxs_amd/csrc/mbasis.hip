@@ -127,20 +127,21 @@ __global__ void __launch_bounds__(WG) mbasis_zband_kernel(BandArgs<T> a, const c
   const uint64_t P = (uint64_t)a.nx * a.ny;
   const uint64_t col = (uint64_t)y * a.nx + (active ? x : 0);
   const cx<T> *base = a.vol[v] + col;
-  // pencil loads first: they stay in flight across the twiddle staging barrier
+  // pencil and band-table loads first: they stay in flight across the twiddle
+  // staging barrier and the FFT
   cx<T> r[E];
 #pragma unroll
   for (int i = 0; i < E; i++) r[i] = base[(uint64_t)F::in_pos(t, i) * P];
+  const uint32_t xx = active ? x : 0;
+  const uint32_t mx = xx ? a.nx - xx : 0, my = y ? a.ny - y : 0;
+  const uint64_t cm = (uint64_t)my * a.nx + mx;
+  const uint32_t b = (uint32_t)a.band[col], bm = (uint32_t)a.band[cm];
   cx<T> *ltw = lds + (size_t)PP * N;
   for (uint32_t i = threadIdx.x; i < (uint32_t)N; i += WG) ltw[i] = tw[i];
   lds_sync();
   F::run(r, t, p, lds, ltw);
   if (!active) return;
-  const uint32_t b = (uint32_t)a.band[col];
   const uint32_t lo = b & 0xffffu, w = b >> 16;
-  const uint32_t mx = x ? a.nx - x : 0, my = y ? a.ny - y : 0;
-  const uint64_t cm = (uint64_t)my * a.nx + mx;
-  const uint32_t bm = (uint32_t)a.band[cm];
   const uint32_t lom = bm & 0xffffu, wm = bm >> 16;
   cx<T> *own = a.own + (uint64_t)(a.vbase + v) * a.J * P;
   cx<T> *mir = a.mir + (uint64_t)(a.vbase + v) * a.J * P;

@@ -110,7 +110,7 @@ __global__ void __launch_bounds__(256) rotate_all_kernel(const T *__restrict__ p
 // q = (q0, q1, q2): I(theta) = c^2 q0 + s^2 q1 + 2 c s q2.  Bilinear sampling is
 // linear, so the three components are sampled (same taps, weights and validity
 // as bilinear_sample; a tap of zero weight contributes nothing, NaN included)
-// and combined once per angle.  Each angle's back-mapped point, taps and
+// and combined once per angle, in the storage precision (FP64 runs: FP64).  Each angle's back-mapped point, taps and
 // weights serve NE energies (grid y = group of NE energies, ne in total).
 template <typename T, int NE>
 __global__ void __launch_bounds__(256) rotate_quad_kernel(const T *__restrict__ quad, const double *__restrict__ inv_tab,
@@ -145,21 +145,30 @@ __global__ void __launch_bounds__(256) rotate_quad_kernel(const T *__restrict__ 
     const uint32_t x1 = x0 + 1 < nx ? x0 + 1 : x0, y1 = y0 + 1 < ny ? y0 + 1 : y0;
     const uint32_t off[4] = {y0 * nx + x0, y0 * nx + x1, y1 * nx + x0, y1 * nx + x1};  // P < 2^32
     const double gx = 1 - fx, gy = 1 - fy;
-    const double w[4] = {gx * gy, fx * gy, gx * fy, fx * fy};
+    const double wd[4] = {gx * gy, fx * gy, gx * fy, fx * fy};
+    // taps are interpolated in the storage precision (the weights rounded once per
+    // angle); which taps count (weight exactly 0 or not) is decided on the FP64
+    // weights, and the sum over angles stays FP64
+    T w[4];
+    bool z[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      w[i] = (T)wd[i];
+      z[i] = wd[i] == 0;
+    }
     const double *c = coef + 3 * (uint64_t)a;
-    const double c0 = c[0], c1 = c[1], c2 = c[2];
+    const T c0 = (T)c[0], c1 = (T)c[1], c2 = (T)c[2];
 #pragma unroll
     for (int e = 0; e < NE; e++) {
-      double s0 = 0, s1 = 0, s2 = 0;
+      T s0 = 0, s1 = 0, s2 = 0;
 #pragma unroll
       for (int i = 0; i < 4; i++) {
         const V4 tp = q4[e][off[i]];
-        const bool z = w[i] == 0;  // selected in the storage type: one move per component
-        s0 += w[i] * (double)(z ? (T)0 : tp.x);
-        s1 += w[i] * (double)(z ? (T)0 : tp.y);
-        s2 += w[i] * (double)(z ? (T)0 : tp.z);
+        s0 += w[i] * (z[i] ? (T)0 : tp.x);
+        s1 += w[i] * (z[i] ? (T)0 : tp.y);
+        s2 += w[i] * (z[i] ? (T)0 : tp.z);
       }
-      const double v = inside ? c0 * s0 + c1 * s1 + c2 * s2 : __longlong_as_double(0x7ff8000000000000ll);
+      const double v = inside ? (double)(c0 * s0 + c1 * s1 + c2 * s2) : __longlong_as_double(0x7ff8000000000000ll);
       if (rot_mask) {
         const bool ok = !isnan(v);
         acc[e] += ok ? v : 0.0;
