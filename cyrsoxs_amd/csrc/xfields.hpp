@@ -198,7 +198,8 @@ __global__ void __launch_bounds__(WG, (MODE == kXTensor && F::kE <= 8) ? 4 : 2)
   const cx<T> *twp = a.twx;
   if constexpr (x_lds_twiddles<MODE>()) {
     cx<T> *ltw = lds + (size_t)a.pp * a.pitch;
-    for (uint32_t i = threadIdx.x; i < a.nx; i += WG) ltw[i] = a.twx[i];
+    // (run-time plans launch pp x tpp <= WG threads: stride by the actual block)
+    for (uint32_t i = threadIdx.x; i < a.nx; i += blockDim.x) ltw[i] = a.twx[i];
     lds_sync();  // (single-wave blocks: an LDS wait only)
     twp = ltw;
   }

@@ -188,6 +188,23 @@ def test_mbasis_matches_ebasis(prec):
     assert err < (1e-11 if prec == capi.FP64 else 2e-5), err
 
 
+def test_mbasis_large_plane_rotation():
+    """A 1152^2 film: quadratic-form planes above 16 MB take the one-energy-per-pass
+    rotation; held against the E-basis executor's per-angle rotation."""
+    n, M = 1152, 2
+    vox = random_euler((n, n, 1), M, 61)
+    energies = [284.0, 286.0, 288.5]
+    mats = random_constants(len(energies), M, 62)
+    out = {}
+    for pipeline in (capi.PIPE_EBASIS, capi.PIPE_MBASIS):
+        sim = capi.Sim((n, n, 1), M, 1.0, energies, (0.0, 5.0, 175.0), interpolation=1, precision=capi.FP32,
+                       pipeline=pipeline)
+        out[pipeline] = capi.simulate(sim, vox, mats)
+    err, same = pattern_rel_l2(out[capi.PIPE_MBASIS], out[capi.PIPE_EBASIS])
+    print(f"large plane: rel L2 {err:.3e}")
+    assert same and err < 1e-4  # two FP32 paths, each held to 1e-4 of the oracle
+
+
 def test_energy_lanes_bitwise():
     """One or two (energy, k) units in flight give bit-identical patterns."""
     c = synthetic.config0()
