@@ -67,6 +67,8 @@ CASES = {
     "five_materials": dict(dims=(24, 24, 16), M=5, morph=1, kw=dict(interpolation=1, reference_frame=0)),
     "deep_z": dict(dims=(32, 24, 128), M=3, morph=0, kw=dict(interpolation=1, windowing=1)),
     "deep_z_nn": dict(dims=(24, 32, 64), M=2, morph=1, kw=dict(interpolation=0)),
+    # run-time X plan (no compile-time codelet for 288) with rows wider than a block
+    "rt_wide_rows": dict(dims=(288, 288, 4), M=3, morph=0, kw=dict(interpolation=1, windowing=1)),
 }
 
 
