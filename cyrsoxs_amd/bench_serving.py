@@ -15,10 +15,10 @@ on the host, measured by the engine.  Reported per rate: p50/p99 latency,
 completed requests, throughput in voxel*angle*energy/s.  The same stream is
 also served FIFO by the plain per-angle executor on one slot at the lowest
 rate (--plain-window), and by the served executor (--pipeline, default the
-material-basis executor) on one slot (FIFO, no overlap) at one rate, for
-comparison.
+choice between the E-basis and material-basis executors per request) on one
+slot (FIFO, no overlap) at one rate, for comparison.
 
-    python -m cyrsoxs_amd.bench_serving [--rates 2,5,10,20,40] [--window 15] [--slots 3] [--pipeline mbasis|ebasis]
+    python -m cyrsoxs_amd.bench_serving [--rates 2,5,10,20,40] [--window 15] [--slots 3] [--pipeline auto|mbasis|ebasis]
 """
 from __future__ import annotations
 
@@ -49,7 +49,7 @@ def main(argv=None):
     ap.add_argument("--plain-window", type=float, default=10.0)
     ap.add_argument("--fifo-rate", type=float, default=5.0)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--pipeline", choices=("mbasis", "ebasis"), default="mbasis")
+    ap.add_argument("--pipeline", choices=("auto", "mbasis", "ebasis"), default="auto")
     a = ap.parse_args(argv)
 
     import numpy as np
@@ -58,7 +58,7 @@ def main(argv=None):
     from cyrsoxs_amd import capi, synthetic
 
     torch.cuda.set_device(0)
-    PIPE = capi.PIPE_MBASIS if a.pipeline == "mbasis" else capi.PIPE_EBASIS
+    PIPE = {"auto": capi.PIPE_AUTO, "mbasis": capi.PIPE_MBASIS, "ebasis": capi.PIPE_EBASIS}[a.pipeline]
     grids = [tuple(int(x) for x in g.split("x")) for g in a.grids.split(",")]
     rates = [float(r) for r in a.rates.split(",")]
 

@@ -21,7 +21,7 @@ STATUS = {0: "CYR_OK", 1: "CYR_E_INVALID", 2: "CYR_E_NOMEM", 3: "CYR_E_DEVICE", 
           9: "CYR_E_NOTFOUND"}
 FP32, FP64 = 0, 1
 STORE_NATIVE, STORE_FP16, STORE_BF16 = 0, 1, 2
-PIPE_PLAIN, PIPE_EBASIS, PIPE_MBASIS = 0, 1, 2
+PIPE_PLAIN, PIPE_EBASIS, PIPE_MBASIS, PIPE_AUTO = 0, 1, 2, 3
 
 _dp = ctypes.POINTER(ctypes.c_double)
 
@@ -108,6 +108,7 @@ _SIGS = {
     "cyr_device_count": (ctypes.c_int, []),
     "cyr_num_angles": (ctypes.c_uint32, [ctypes.c_double, ctypes.c_double, ctypes.c_double]),
     "cyr_footprint_bytes": (ctypes.c_uint64, [ctypes.c_void_p]),
+    "cyr_resolved_pipeline": (ctypes.c_int32, [ctypes.c_void_p]),
     "cyr_release_workspace": (ctypes.c_int, []),
     "cyr_version": (ctypes.c_char_p, []),
     "cyr_kernel_timing": (ctypes.c_int, [ctypes.c_int]),
@@ -201,6 +202,10 @@ class Sim:
 
     def footprint(self):
         return int(lib().cyr_footprint_bytes(self.ref))
+
+    def resolved_pipeline(self):
+        """The executor this simulation runs (PIPE_AUTO resolved)."""
+        return int(lib().cyr_resolved_pipeline(self.ref))
 
 
 def _ptr(a):

@@ -38,10 +38,10 @@ namespace {
 // Relative cost of one simulation, for scheduling: HBM bytes the executor moves.
 double sim_cost(const cyr_sim_desc *d, const SimGeometry &g) {
   const double V = (double)g.V, M = g.M, units = (double)g.nE * g.nK;
-  const bool fused = d->pipeline != CYR_PIPE_PLAIN && ebasis_usable(d, g);
-  if (fused && d->pipeline == CYR_PIPE_MBASIS)  // basis passes per k, then 2D work per energy
+  const int pipe = resolved_pipeline(d, g);
+  if (pipe == CYR_PIPE_MBASIS)  // basis passes per k, then 2D work per energy
     return g.nK * V * (16.0 * M + 7.0 * M * 8.0 * 6.0) + units * g.P * (16.0 * 7.0 * M + 8.0 * g.nA);
-  if (fused) return units * V * (16.0 * M + 80.0);
+  if (pipe == CYR_PIPE_EBASIS) return units * V * (16.0 * M + 80.0);
   return units * g.nA * V * (16.0 * M + 24.0 + 96.0);
 }
 
@@ -52,7 +52,7 @@ cyr_status tune_shapes(uint32_t count, const cyr_sim_desc *descs, const std::vec
   for (uint32_t i = 0; i < count; i++) {
     const cyr_sim_desc *d = &descs[i];
     const SimGeometry &g = geo[i];
-    if (d->pipeline != CYR_PIPE_EBASIS || !ebasis_usable(d, g) || g.nz < 2) continue;
+    if (resolved_pipeline(d, g) != CYR_PIPE_EBASIS || g.nz < 2) continue;
     const auto key = std::make_tuple(g.nx, g.ny, g.nz, d->precision);
     if (!seen.insert(key).second) continue;
     const YzTuning t = ebasis_yz_tuning(g.nx, g.ny, g.nz, d->precision);

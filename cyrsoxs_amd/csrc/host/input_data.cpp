@@ -242,7 +242,7 @@ void InputData::print(std::ostream &os) const {
   os << "Algorithm            : " << alg_name[algorithmType % 2] << "\n";
   os << "Reference Frame      : " << frame_name[referenceFrame ? 1 : 0] << "(" << referenceFrame << ")\n";
   os << "Precision            : " << (precision == CYR_FP64 ? "FP64" : "FP32") << "\n";
-  os << "Pipeline             : " << (pipeline == CYR_PIPE_EBASIS ? "E-basis" : pipeline == CYR_PIPE_MBASIS ? "material basis" : "plain") << "\n";
+  os << "Pipeline             : " << (pipeline == CYR_PIPE_EBASIS ? "E-basis" : pipeline == CYR_PIPE_MBASIS ? "material basis" : pipeline == CYR_PIPE_AUTO ? "auto (E-basis or material basis)" : "plain") << "\n";
 }
 
 void InputData::printToFile(std::ostream &os) const {

@@ -50,7 +50,7 @@ struct InputData {
   int NUM_MATERIAL = 0;
   // runtime switches that are compile-time in the reference
   int precision = CYR_FP32;
-  int pipeline = CYR_PIPE_EBASIS;
+  int pipeline = CYR_PIPE_AUTO;
 
   bool if2DComputation() const { return voxelDims[2] == 1; }
   uint64_t numVoxels() const { return (uint64_t)voxelDims[0] * voxelDims[1] * voxelDims[2]; }

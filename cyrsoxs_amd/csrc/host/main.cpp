@@ -1,6 +1,6 @@
 // CyRSoXS command-line executable (reference: src/main.cpp:107-173).
 //
-//   CyRSoXS <morphology.h5> [HDF5OutputDir] [--precision fp32|fp64] [--pipeline ebasis|mbasis|plain]
+//   CyRSoXS <morphology.h5> [HDF5OutputDir] [--precision fp32|fp64] [--pipeline auto|ebasis|mbasis|plain]
 //
 // Reads config.txt and Material<N>.txt from the working directory, the
 // morphology from the HDF5 file, runs the pipeline on the current HIP device
@@ -27,14 +27,14 @@ int main(int argc, char **argv) {
       in.precision = (p == "fp64" || p == "double") ? CYR_FP64 : CYR_FP32;
     } else if (a == "--pipeline" && i + 1 < argc) {
       std::string p = argv[++i];
-      in.pipeline = p == "plain" ? CYR_PIPE_PLAIN : (p == "mbasis" ? CYR_PIPE_MBASIS : CYR_PIPE_EBASIS);
+      in.pipeline = p == "plain" ? CYR_PIPE_PLAIN : p == "mbasis" ? CYR_PIPE_MBASIS : p == "ebasis" ? CYR_PIPE_EBASIS : CYR_PIPE_AUTO;
     } else {
       pos.push_back(a);
     }
   }
   if (pos.empty()) {
     std::cout << "Usage : " << argv[0] << " HDF5FileName HDF5OutputDirname [optional]"
-              << " [--precision fp32|fp64] [--pipeline ebasis|mbasis|plain]\n";
+              << " [--precision fp32|fp64] [--pipeline auto|ebasis|mbasis|plain]\n";
     return 1;
   }
   const std::string fname = pos[0];

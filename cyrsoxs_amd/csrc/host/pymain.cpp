@@ -5,8 +5,9 @@
 //
 // Additions (no reference counterpart): InputData.precision (Precision.FP32/FP64,
 // the reference's compile-time DOUBLE_PRECISION switch) and InputData.pipeline
-// (Pipeline.EBasis default, Pipeline.MBasis = material-basis spectra, Pipeline.Plain =
-// per-angle reference structure).
+// (Pipeline.Auto default = E-basis or material basis by cost, Pipeline.EBasis,
+// Pipeline.MBasis = material-basis spectra, Pipeline.Plain = per-angle reference
+// structure).
 #include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
 #include <pybind11/stl.h>
@@ -43,7 +44,7 @@ enum class Case : int { Default = 0, BeamDivergence = 1, GrazingIncidence = 2 };
 enum class Frame : int { Material = 0, Lab = 1 };
 enum class Order : int { ZYX = 0, XYZ = 1 };
 enum class Precision : int { FP32 = CYR_FP32, FP64 = CYR_FP64 };
-enum class Pipeline : int { Plain = CYR_PIPE_PLAIN, EBasis = CYR_PIPE_EBASIS, MBasis = CYR_PIPE_MBASIS };
+enum class Pipeline : int { Plain = CYR_PIPE_PLAIN, EBasis = CYR_PIPE_EBASIS, MBasis = CYR_PIPE_MBASIS, Auto = CYR_PIPE_AUTO };
 
 int to_int(const py::object &o) { return py::int_(o).cast<int>(); }
 
@@ -557,7 +558,7 @@ PYBIND11_MODULE(CyRSoXS, module) {
   py::enum_<Frame>(module, "ReferenceFrame").value("Lab", Frame::Lab).value("Material", Frame::Material).export_values();
   py::enum_<Order>(module, "MorphologyOrder").value("XYZ", Order::XYZ).value("ZYX", Order::ZYX).export_values();
   py::enum_<Precision>(module, "Precision").value("FP32", Precision::FP32).value("FP64", Precision::FP64).export_values();
-  py::enum_<Pipeline>(module, "Pipeline").value("Plain", Pipeline::Plain).value("EBasis", Pipeline::EBasis).value("MBasis", Pipeline::MBasis).export_values();
+  py::enum_<Pipeline>(module, "Pipeline").value("Plain", Pipeline::Plain).value("EBasis", Pipeline::EBasis).value("MBasis", Pipeline::MBasis).value("Auto", Pipeline::Auto).export_values();
 
   py::class_<PyInputData>(module, "InputData")
       .def(py::init<int>(), "Constructor", py::arg("NumMaterial"))
@@ -602,9 +603,9 @@ PYBIND11_MODULE(CyRSoXS, module) {
           "pipeline", [](const PyInputData &s) { return s.d.pipeline; },
           [](PyInputData &s, py::object v) {
             const int p = to_int(v);
-            s.d.pipeline = p == CYR_PIPE_PLAIN ? CYR_PIPE_PLAIN : (p == CYR_PIPE_MBASIS ? CYR_PIPE_MBASIS : CYR_PIPE_EBASIS);
+            s.d.pipeline = (p >= CYR_PIPE_PLAIN && p <= CYR_PIPE_AUTO) ? p : CYR_PIPE_AUTO;
           },
-          "execution strategy (Pipeline.EBasis / Pipeline.MBasis / Pipeline.Plain)");
+          "execution strategy (Pipeline.Auto default / EBasis / MBasis / Plain)");
 
   py::class_<PyRefractiveIndex>(module, "RefractiveIndex")
       .def(py::init<PyInputData &>(), "Constructor", py::keep_alive<1, 2>())

@@ -194,7 +194,7 @@ cyr_status validate_desc(const cyr_sim_desc *d) {
   if (d->algorithm < 0 || d->algorithm > 1) return fail(CYR_E_INVALID, "algorithm out of range");
   if (d->precision != CYR_FP32 && d->precision != CYR_FP64) return fail(CYR_E_INVALID, "precision out of range");
   if (d->storage < 0 || d->storage > 2) return fail(CYR_E_INVALID, "storage out of range");
-  if (d->pipeline < 0 || d->pipeline > 2) return fail(CYR_E_INVALID, "pipeline out of range");
+  if (d->pipeline < 0 || d->pipeline > 3) return fail(CYR_E_INVALID, "pipeline out of range");
   if (d->case_type != 0 && (d->num_kvectors < 1 || !d->kvectors))
     return fail(CYR_E_INVALID, "case types 1 and 2 need at least one k vector");
   if (d->num_kvectors > 0 && !d->kvectors) return fail(CYR_E_INVALID, "kvectors is null");

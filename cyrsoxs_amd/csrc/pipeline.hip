@@ -35,6 +35,23 @@ bool ebasis_usable(const cyr_sim_desc *d, const SimGeometry &g) {
          !(d->precision == CYR_FP64 && d->storage != CYR_STORE_NATIVE);
 }
 
+// HBM bytes per voxel (E-basis: per energy and k vector; material basis: per k
+// vector) and per pixel and energy.  The E-basis Y/z kernel is ALU-bound at about
+// twice its bytes, so its reads count double.  Measured at 512^2 x 128, 4
+// materials: E-basis 1.26 ms per energy, material basis 3.7 ms per k + 0.05 ms
+// per energy; the model's break-even (2.8 energies) matches.
+int resolved_pipeline(const cyr_sim_desc *d, const SimGeometry &g) {
+  if (d->pipeline == CYR_PIPE_PLAIN || !ebasis_usable(d, g)) return CYR_PIPE_PLAIN;
+  if (d->pipeline != CYR_PIPE_AUTO) return d->pipeline;
+  const double cs = d->precision == CYR_FP64 ? 16.0 : 8.0;
+  const double el = d->storage == CYR_STORE_NATIVE ? cs / 2 : 2.0;
+  const double nv = mbasis_volumes(g.M), ng = (g.nz > 1 && d->interpolation == 1) ? 2.0 : 1.0;
+  const double morph = 4.0 * el * g.M;
+  const double eb = (double)g.nE * (double)g.V * (morph + 15.0 * cs);
+  const double mb = (double)g.V * (morph + 4.0 * cs * nv) + (double)g.nE * (double)g.P * 4.0 * ng * nv * cs;
+  return mb < eb ? CYR_PIPE_MBASIS : CYR_PIPE_EBASIS;
+}
+
 namespace {
 
 template <typename U>
@@ -675,10 +692,10 @@ cyr_status run_impl(const cyr_sim_desc *d, const SimGeometry &g, const void *d_v
   }
   Setup<T> su{};
   CYR_TRY(setup_views<T>(d, g, tb, tables, su));
-  const bool fused = d->pipeline != CYR_PIPE_PLAIN && ebasis_usable(d, g);
-  const cyr_status s = !fused ? run_plain<T>(d, g, d_vox, d_out, wb, st, su)
-                       : d->pipeline == CYR_PIPE_MBASIS ? run_mbasis<T>(d, g, d_vox, d_out, wb, st, su)
-                                                       : run_ebasis<T>(d, g, d_vox, d_out, wb, st, su);
+  const int pipe = resolved_pipeline(d, g);
+  const cyr_status s = pipe == CYR_PIPE_PLAIN ? run_plain<T>(d, g, d_vox, d_out, wb, st, su)
+                       : pipe == CYR_PIPE_MBASIS ? run_mbasis<T>(d, g, d_vox, d_out, wb, st, su)
+                                                 : run_ebasis<T>(d, g, d_vox, d_out, wb, st, su);
   if (mode != kRunSync) return s;
   // drain the stream on errors too, so no queued launch outlives the call's buffers
   const hipError_t se = hipStreamSynchronize(st);
@@ -807,9 +824,9 @@ uint64_t output_bytes(const cyr_sim_desc *d, const SimGeometry &g) {
 
 uint64_t work_bytes(const cyr_sim_desc *d, const SimGeometry &g) {
   const uint64_t cs = d->precision == CYR_FP64 ? 16 : 8, rs = cs / 2;
-  const bool fused = d->pipeline != CYR_PIPE_PLAIN && ebasis_usable(d, g);
+  const int pipe = resolved_pipeline(d, g);
   uint64_t b = 0;
-  if (fused && d->pipeline == CYR_PIPE_MBASIS) {
+  if (pipe == CYR_PIPE_MBASIS) {
     // one pass of packed volumes, band values (own + mirror, at most the band cap),
     // band table, DC sums, per-angle projections, average
     const uint64_t nv = mbasis_volumes(g.M), pass = nv < 7 ? nv : 7;
@@ -818,7 +835,7 @@ uint64_t work_bytes(const cyr_sim_desc *d, const SimGeometry &g) {
     const uint64_t planes = d->reference_frame != 0 ? 4 * nb : g.nA;
     b += cs * g.V * pass + cs * 2 * nv * J * g.P + 4 * g.P + 4 * mbasis_bmax_words(g.nx, g.ny) +
          16 * nv * (g.nz + 1) + rs * g.P * planes + 8 * g.P * nb;
-  } else if (fused) {  // per lane: fields, bin table, Y/z partial sums (chunks x 2 bins x fields), DC sums,
+  } else if (pipe == CYR_PIPE_EBASIS) {  // per lane: fields, bin table, Y/z partial sums (chunks x 2 bins x fields), DC sums,
                  // per-angle projections, average
     const uint64_t nch = ebasis_max_chunks(g.nx, g.ny, g.nz, d->precision);
     const uint64_t lanes = (uint64_t)g.nE * g.nK > 1 ? energy_lanes() : 1;
@@ -1006,6 +1023,12 @@ uint64_t cyr_footprint_bytes(const cyr_sim_desc *desc) {
   SimGeometry g;
   if (build_geometry(desc, g) != CYR_OK) return 0;
   return morphology_bytes(desc) + work_bytes(desc, g) + output_bytes(desc, g);
+}
+
+int32_t cyr_resolved_pipeline(const cyr_sim_desc *desc) {
+  SimGeometry g;
+  if (build_geometry(desc, g) != CYR_OK) return -1;
+  return resolved_pipeline(desc, g);
 }
 
 cyr_status cyr_simulate_device(const cyr_sim_desc *desc, const void *d_voxels, const double *mats, void *d_out,

@@ -74,6 +74,9 @@ void fill_tables(const cyr_sim_desc *d, const SimGeometry &g, const double *mats
 
 // True when the simulation runs the fused E-basis kernels (else the plain executor).
 bool ebasis_usable(const cyr_sim_desc *d, const SimGeometry &g);
+// The executor a simulation runs: CYR_PIPE_PLAIN, CYR_PIPE_EBASIS or CYR_PIPE_MBASIS
+// (CYR_PIPE_AUTO resolved by HBM bytes; fused executors need ebasis_usable).
+int resolved_pipeline(const cyr_sim_desc *d, const SimGeometry &g);
 
 // Autotunes the E-basis Y/z launch shape of the simulation's grid (see cyr_autotune).
 struct YzTuning;

@@ -54,7 +54,9 @@ enum { CYR_STORE_NATIVE = 0, CYR_STORE_FP16 = 1, CYR_STORE_BF16 = 2 };
  * forms each energy's fields there as combinations with that energy's
  * constants; the per-energy work is then 2D.  All three produce the same
  * patterns within floating-point rounding. */
-enum { CYR_PIPE_PLAIN = 0, CYR_PIPE_EBASIS = 1, CYR_PIPE_MBASIS = 2 };
+enum { CYR_PIPE_PLAIN = 0, CYR_PIPE_EBASIS = 1, CYR_PIPE_MBASIS = 2, CYR_PIPE_AUTO = 3 };
+/* CYR_PIPE_AUTO picks EBASIS or MBASIS per simulation from the HBM bytes each would
+ * move (MBASIS once a simulation has more than a few energies per k vector). */
 
 /* Mirrors InputData (include/Input/InputData.h:186-232) plus the compile-time
  * switches of the reference build.  Enumerations use the reference's values
@@ -248,6 +250,10 @@ const char *cyr_last_error(void);
 int cyr_device_count(void);
 /* numAnglesRotation = round((end - start)/increment + 1) (src/cudaMain.cu:393-394). */
 uint32_t cyr_num_angles(double start, double increment, double end);
+/* The executor a descriptor runs: CYR_PIPE_PLAIN, CYR_PIPE_EBASIS or
+ * CYR_PIPE_MBASIS (CYR_PIPE_AUTO resolved; grids without fused-kernel plans run
+ * PLAIN); -1 for an invalid descriptor.  No counterpart in the reference. */
+int32_t cyr_resolved_pipeline(const cyr_sim_desc *desc);
 /* Device bytes a simulation holds while it runs (morphology + work volumes). */
 uint64_t cyr_footprint_bytes(const cyr_sim_desc *desc);
 /* Frees the per-device cached work buffers of the plain/E-basis executors. */

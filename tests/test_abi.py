@@ -113,6 +113,31 @@ def test_footprint_accounts_for_pipeline():
     assert plain.footprint() - half.footprint() == V * 4 * 8
 
 
+def test_footprint_material_basis():
+    """Material basis at 512^2 x 128, 4 materials: one pass of 7 packed complex64
+    volumes plus band values, besides the morphology."""
+    mb = _sim(dims=(512, 512, 128), num_materials=4, pipeline=capi.PIPE_MBASIS)
+    V = 512 * 512 * 128
+    assert mb.footprint() >= V * (16 * 4 + 7 * 8)
+
+
+def test_auto_pipeline_resolution():
+    """AUTO runs the E-basis executor for a single energy and the material basis for
+    many; explicit choices are kept; grids without fused plans run plain."""
+    one = _sim(dims=(512, 512, 128), num_materials=4, pipeline=capi.PIPE_AUTO, energies=[285.0])
+    many = _sim(dims=(512, 512, 128), num_materials=4, pipeline=capi.PIPE_AUTO,
+                energies=list(np.linspace(280, 299, 20)))
+    assert one.resolved_pipeline() == capi.PIPE_EBASIS
+    assert many.resolved_pipeline() == capi.PIPE_MBASIS
+    for p in (capi.PIPE_PLAIN, capi.PIPE_EBASIS, capi.PIPE_MBASIS):
+        assert _sim(dims=(64, 64, 32), num_materials=2, pipeline=p).resolved_pipeline() == p
+    # 7 = a prime without a Stockham plan: every request runs the plain executor
+    assert _sim(dims=(7, 7, 7), num_materials=2, pipeline=capi.PIPE_MBASIS).resolved_pipeline() == capi.PIPE_PLAIN
+    bad = _sim(dims=(64, 64, 32), num_materials=2, pipeline=capi.PIPE_AUTO)
+    bad.desc.pipeline = 9
+    assert bad.resolved_pipeline() == -1
+
+
 def test_no_device_reported_without_gpu():
     torch = pytest.importorskip("torch")
     if torch.cuda.is_available():

@@ -36,7 +36,7 @@ def _check(got, ref, prec):
 PIPES = [capi.PIPE_PLAIN, capi.PIPE_EBASIS, capi.PIPE_MBASIS]
 
 
-@pytest.mark.parametrize("pipeline", PIPES)
+@pytest.mark.parametrize("pipeline", PIPES + [capi.PIPE_AUTO])
 @pytest.mark.parametrize("prec", [capi.FP64, capi.FP32])
 def test_sphere_64(prec, pipeline):
     """The 64^3 two-material sphere, 3 energies x 6 angles, nearest neighbour."""
