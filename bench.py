@@ -32,8 +32,8 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this
 # workload (material-basis executor), scaled by the counters' reading for a 1 GiB
 # negation (fetch x2.0, write x1.0); absent where not measured.
-TRAFFIC = {"mbasis": {"xfields": 2.954e9, "fft_y": 3.758e9, "mbasis_zband": 2.017e9, "mbasis_quad": 3.338e9,
-                      "rotate_quad": 1.248e9}}
+TRAFFIC = {"mbasis": {"xfields": 2.954e9, "fft_y": 3.758e9, "mbasis_zband": 2.017e9, "mbasis_quad": 0.375e9,
+                      "rotate_quad": 1.087e9}}
 # timing labels of the library (cyr_kernel_timing) -> kernel symbol
 KERNEL_OF = {"xfields": "xfields_kernel", "fft_y": "fft_ct_kernel (Y pass)", "fft_z": "fft_ct_kernel (Z pass)",
              "mbasis_extract": "mbasis_extract_kernel", "mbasis_zband": "mbasis_zband_kernel", "mbasis_angles": "mbasis_angles_kernel",

@@ -648,7 +648,9 @@ cyr_status run_mbasis(const cyr_sim_desc *d, const SimGeometry &g, const void *d
           aa.quad = proj;
           kt_begin(s);
           CYR_TRY(launch_mbasis_angles<T>(aa, ng, s));
-          kt_end("mbasis_quad", s, n * (2.0 * ng * nvol * (double)P * cxb + 4.0 * P * sizeof(T)));
+          // unique inputs: the group's band values (own + mirror, at most J slots per
+          // column, shared by its energies); outputs: n quadratic-form planes
+          kt_end("mbasis_quad", s, 2.0 * nvol * (double)J * P * cxb + n * 4.0 * P * sizeof(T));
           kt_begin(s);
           CYR_TRY(launch_rotate_quad<T>(proj, inv, su.quad + (size_t)3 * k * g.nA, g.nA, g.nx, g.ny, d->rot_mask, avg,
                                         n, su.tiles, s));
