@@ -2,12 +2,10 @@
 //
 // A block holds pp rows of tpp threads (F::kE elements each); WG = 64 makes
 // every row exchange wave-local (the barriers compile to LDS waits).  LDS: the
-// pp x row tile, plus (basis modes) the X twiddle table.  The tensor mode reads
-// its twiddles from the global table (5 FFTs per row: staging them per block cost
-// more than it saved, as did looping a block over several rows, which spills
-// registers); the basis modes' 4-7 FFTs per row read them from LDS (the pair
-// kernel's vector-memory instructions drop from 124 to ~34 per wave: 731 ->
-// 621 us at 512^2 x 128).
+// pp x row tile, plus the X twiddle table: the 4-7 FFTs per row read their
+// twiddles from LDS (the pair kernel's vector-memory instructions drop from 124 to
+// 34 per wave: 731 -> 621 us at 512^2 x 128; the tensor kernel 780 -> 742 us).
+// Looping a block over several rows spills registers.
 //
 // What a voxel contributes is selected by MODE:
 //   kXTensor  the five E-basis tensor fields N'00, N'01, N'02, N'11, N'12 of one
@@ -39,9 +37,8 @@
 namespace cyr {
 
 enum XMode { kXTensor = 0, kXPair = 1, kXSingle = 2 };
-// Basis modes run 4-7 FFTs per row: their X twiddles are staged in LDS once per
-// block (the tensor mode's 5 FFTs measured faster reading the global table).
-template <int MODE> constexpr bool x_lds_twiddles() { return MODE != kXTensor; }
+// Every mode runs 4-7 FFTs per row: the X twiddles are staged in LDS once per block.
+template <int MODE> constexpr bool x_lds_twiddles() { return true; }
 
 // Compile-time loop: fn(std::integral_constant<int, I>) for I in [I0, N).  Used
 // where a #pragma unroll may be declined (large bodies), which would turn
