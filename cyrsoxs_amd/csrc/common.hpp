@@ -56,6 +56,54 @@ template <typename T> __device__ inline cx<T> cmul(cx<T> a, cx<T> b) {
   return mk<T>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 template <typename T> __device__ inline cx<T> cscale(cx<T> a, T s) { return mk<T>(a.x * s, a.y * s); }
+
+// Twiddle tables of length n hold w^m = exp(-2 pi i m / n), m < n, computed in
+// FP64 and rounded once.  An FP32 table has a second block of n entries right
+// after it: the residuals lo[m] = w^m - hi[m], rounded to FP32, so that hi + lo
+// carries the FP64 twiddle to about 48 bits.  tw_len is the table's entry count
+// (what an LDS copy of the table must hold).
+template <typename T> __host__ __device__ constexpr uint32_t tw_blocks() { return sizeof(T) == 4 ? 2u : 1u; }
+template <typename T> __host__ __device__ constexpr size_t tw_len(size_t n) { return n * tw_blocks<T>(); }
+
+// Blocks of wg threads using `bytes` of LDS that one CU holds at once
+// (160 KiB of LDS and 32 waves per CU).
+__host__ __device__ constexpr unsigned cu_blocks(size_t bytes, unsigned wg) {
+  const unsigned by_waves = 32u / ((wg + 63u) / 64u);
+  const size_t by_lds = bytes ? (size_t)163840 / bytes : by_waves;
+  return by_lds < by_waves ? (unsigned)by_lds : by_waves;
+}
+// Whether a kernel with `used` LDS entries besides a staged length-n twiddle
+// copy can also stage the FP32 residual block without losing occupancy
+// (otherwise the residuals are read from the global table).
+template <typename T> __host__ __device__ constexpr bool tw_lo_in_lds(size_t used, size_t n, unsigned wg) {
+  return tw_blocks<T>() == 2 &&
+         cu_blocks((used + 2 * n) * sizeof(cx<T>), wg) >= cu_blocks((used + n) * sizeof(cx<T>), wg);
+}
+
+// Compensated twiddle product v (hi + lo): the small product v lo is formed
+// first and folded into the fused multiply-adds of v hi, so the twiddle's own
+// rounding error (up to 2^-24 relative) drops out and only the arithmetic's
+// remains.  FP64 tables have no residual block and take the plain product.
+template <typename T> __device__ inline cx<T> cmul_comp(cx<T> v, cx<T> hi, cx<T> lo) {
+  if constexpr (sizeof(T) == 4) {
+    const float cr = v.x * lo.x - v.y * lo.y, ci = v.x * lo.y + v.y * lo.x;
+    return mk<T>(__builtin_fmaf(v.x, hi.x, __builtin_fmaf(-v.y, hi.y, cr)),
+                 __builtin_fmaf(v.x, hi.y, __builtin_fmaf(v.y, hi.x, ci)));
+  } else {
+    (void)lo;
+    return cmul<T>(v, hi);
+  }
+}
+// Re w^e of a length-n table in FP64 (hi + lo for FP32 tables).
+template <typename T> __device__ inline double tw_cos(const cx<T> *tw, uint32_t n, uint32_t e) {
+  if constexpr (sizeof(T) == 4) return (double)tw[e].x + (double)tw[n + e].x;
+  else return (double)tw[e].x;
+}
+// v * w^e from a table of length n (tw: hi block; the residual block follows at tw + n).
+template <typename T> __device__ inline cx<T> cmul_tw(cx<T> v, const cx<T> *tw, uint32_t n, uint32_t e) {
+  if constexpr (sizeof(T) == 4) return cmul_comp<T>(v, tw[e], tw[n + e]);
+  else return cmul<T>(v, tw[e]);
+}
 // multiply by -i
 template <typename T> __device__ inline cx<T> cmul_mi(cx<T> a) { return mk<T>(a.y, -a.x); }
 

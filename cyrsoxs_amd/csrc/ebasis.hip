@@ -95,7 +95,7 @@ __global__ void __launch_bounds__(WG) ebasis_yz_kernel(StagePlan P, YzArgs<T> a)
     for (int i = 0; i < E; i++) {
       v[i] = active ? vol[(uint64_t)fft.in_pos(t, i) * a.nx] : mk<T>(0, 0);
     }
-    fft.run(v, t, p, lds, a.twy);
+    fft.template run<false>(v, t, p, lds, a.twy, nullptr);
     if (active) {
 #pragma unroll
       for (int i = 0; i < E; i++) a.part[(uint64_t)c * P2 + (uint64_t)fft.out_pos(t, i) * a.nx + kx] = v[i];
@@ -103,7 +103,8 @@ __global__ void __launch_bounds__(WG) ebasis_yz_kernel(StagePlan P, YzArgs<T> a)
     return;
   } else {
     // LDS: [pp x ny tile][ny Y twiddles][nz z twiddles]; the Y twiddles are
-    // reread every slice, so they are staged next to the tile
+    // reread every slice, so they are staged next to the tile.  This kernel is
+    // VALU-bound: its Y FFT and z DFT keep plain twiddle products.
     cx<T> *lty = lds + (size_t)pp * a.ny;
     cx<T> *ltw = lty + a.ny;
     for (uint32_t i = threadIdx.x; i < a.ny; i += blockDim.x) lty[i] = a.twy[i];
@@ -136,7 +137,7 @@ __global__ void __launch_bounds__(WG) ebasis_yz_kernel(StagePlan P, YzArgs<T> a)
 #pragma unroll
       for (int i = 0; i < E; i++) v[i] = vn[i];
       if (z + 1 < ze) load(z + 1, vn);
-      fft.run(v, t, p, lds, lty);
+      fft.template run<false>(v, t, p, lds, lty, nullptr);
       cx<T> wz = mk<T>(1, 0);  // W^(-z): second bin of the linear pair
       if constexpr (NG == 2) wz = ltw[z == 0 ? 0 : a.nz - z];
 #pragma unroll
@@ -180,12 +181,13 @@ __global__ void __launch_bounds__(256) ebasis_dc_kernel(DcArgs<T> a) {
   __shared__ double red[2][256];
   const uint32_t z = blockIdx.x, c = blockIdx.y;
   const cx<T> *G = a.vol[c] + (uint64_t)z * a.nx * a.ny;
-  const double cz = a.nz > 1 ? 2.0 * (double)a.twz[z].x : 0.0;
+  // cosines from the twiddle tables, FP32 residual added back
+  const double cz = a.nz > 1 ? 2.0 * tw_cos<T>(a.twz, a.nz, z) : 0.0;
   double sr = 0, si = 0;
   for (uint32_t y = threadIdx.x; y < a.ny; y += blockDim.x) {
     const cx<T> *row = G + (uint64_t)y * a.nx;
     const cx<T> g0 = row[0], g1 = row[1], gm = row[a.nx - 1];
-    const double w = 2.0 * (double)a.twy[y].x + cz;
+    const double w = 2.0 * tw_cos<T>(a.twy, a.ny, y) + cz;
     sr += (double)g1.x + (double)gm.x + w * (double)g0.x;
     si += (double)g1.y + (double)gm.y + w * (double)g0.y;
   }

@@ -81,8 +81,13 @@ __device__ inline void run_stage(cx<T> (&v)[E], const StagePlan &P, uint32_t s, 
     const uint32_t j = t + b * tpp;
     const uint32_t k = j % Ls;
     if (k != 0) {
+      if (last && tw_blocks<T>() == 2) {  // finest twiddles: compensated (hi + lo)
 #pragma unroll
-      for (int r = 1; r < R; r++) v[b * R + r] = cmul<T>(v[b * R + r], tw[r * k * tstep]);
+        for (int r = 1; r < R; r++) v[b * R + r] = cmul_tw<T>(v[b * R + r], tw, n, r * k * tstep);
+      } else {
+#pragma unroll
+        for (int r = 1; r < R; r++) v[b * R + r] = cmul<T>(v[b * R + r], tw[r * k * tstep]);
+      }
     }
     bfly<T, R>(&v[b * R]);
   }
@@ -158,6 +163,7 @@ __global__ void __launch_bounds__(1024) fft_pencil_kernel(StagePlan P, PassArgs 
 template <typename T, class F, bool CONTIG, int WG>
 __global__ void __launch_bounds__(WG) fft_ct_kernel(PassArgs A, const cx<T> *__restrict__ tw) {
   constexpr int E = F::kE, TPP = F::TPP, N = TPP * E;
+  constexpr bool LO = !CONTIG && tw_lo_in_lds<T>((size_t)F::PE * N, N, WG);
   extern __shared__ unsigned char smem_raw[];
   cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
   const cx<T> *twp = tw;
@@ -189,13 +195,15 @@ __global__ void __launch_bounds__(WG) fft_ct_kernel(PassArgs A, const cx<T> *__r
 #pragma unroll
   for (int i = 0; i < E; i++) v[i] = active ? gbase[(uint64_t)F::in_pos(t, i) * A.elem_stride] : mk<T>(0, 0);
   if constexpr (!CONTIG) {
-    // twiddles staged while the pencil loads are in flight
+    // twiddles staged while the pencil loads are in flight, with the last
+    // stage's FP32 residuals when they fit without costing occupancy (else they
+    // are read from the global table, L2 resident)
     cx<T> *ltw = lds + (size_t)A.pp * N;
-    for (uint32_t i = threadIdx.x; i < (uint32_t)N; i += WG) ltw[i] = tw[i];
+    for (uint32_t i = threadIdx.x; i < (uint32_t)(LO ? 2 * N : N); i += WG) ltw[i] = tw[i];
     lds_sync();
     twp = ltw;
   }
-  F::run(v, t, p, lds, twp);
+  F::run(v, t, p, lds, twp, LO ? twp + N : tw + N);
   if (active) {
 #pragma unroll
     for (int i = 0; i < E; i++) gbase[(uint64_t)F::out_pos(t, i) * A.elem_stride] = v[i];
@@ -225,7 +233,8 @@ hipError_t launch_ct_n(bool contig, uint32_t nc, PassArgs &A, hipStream_t stream
     A.nvol = nc;
     const uint64_t tiles = (uint64_t)A.groups * A.outer_count * nc;
     if (tiles > 0x7ffffff0ull) return hipErrorInvalidConfiguration;
-    const size_t lds = ((size_t)PP * N + N) * sizeof(cx<T>);
+    constexpr bool LO = tw_lo_in_lds<T>((size_t)PP * N, N, WG);
+    const size_t lds = ((size_t)PP * N + (LO ? 2 * N : N)) * sizeof(cx<T>);
     hipLaunchKernelGGL((fft_ct_kernel<T, F, false, WG>), dim3(xcd_grid(tiles)), dim3(WG), lds, stream, A, tw);
   }
   return hipGetLastError();
@@ -267,9 +276,9 @@ __global__ void __launch_bounds__(256) dft_pencil_kernel(uint32_t n, PassArgs A,
     T re = 0, im = 0;
     uint32_t e = 0;
     for (uint32_t i = 0; i < n; i++) {
-      const cx<T> w = tw[e];
-      re += x[i].x * w.x - x[i].y * w.y;
-      im += x[i].x * w.y + x[i].y * w.x;
+      const cx<T> u = cmul_tw<T>(x[i], tw, n, e);
+      re += u.x;
+      im += u.y;
       e += k;
       if (e >= n) e -= n;
     }
@@ -383,20 +392,27 @@ template <typename T> const cx<T> *twiddles(uint32_t n, cyr_status *st) {
     *st = CYR_OK;
     return reinterpret_cast<const cx<T> *>(it->second.ptr);
   }
-  std::vector<cx<T>> h(n);
+  const size_t len = tw_len<T>(n);
+  std::vector<cx<T>> h(len);
   for (uint32_t m = 0; m < n; m++) {
-    // exact-angle FP64 twiddle, rounded once to the compute precision
+    // exact-angle FP64 twiddle, rounded once to the compute precision; FP32
+    // tables also keep the rounding residual (block 2)
     const double a = 2.0 * M_PI * (double)m / (double)n;
-    h[m].x = (T)std::cos(a);
-    h[m].y = (T)(-std::sin(a));
+    const double c = std::cos(a), sn = -std::sin(a);
+    h[m].x = (T)c;
+    h[m].y = (T)sn;
+    if (tw_blocks<T>() == 2) {
+      h[n + m].x = (T)(c - (double)h[m].x);
+      h[n + m].y = (T)(sn - (double)h[m].y);
+    }
   }
   DeviceBuffer b;
-  cyr_status s = ensure_buffer(b, sizeof(cx<T>) * n);
+  cyr_status s = ensure_buffer(b, sizeof(cx<T>) * len);
   if (s != CYR_OK) {
     *st = s;
     return nullptr;
   }
-  if (hipMemcpy(b.ptr, h.data(), sizeof(cx<T>) * n, hipMemcpyHostToDevice) != hipSuccess) {
+  if (hipMemcpy(b.ptr, h.data(), sizeof(cx<T>) * len, hipMemcpyHostToDevice) != hipSuccess) {
     free_buffer(b);
     *st = fail(CYR_E_DEVICE, "twiddle upload failed");
     return nullptr;

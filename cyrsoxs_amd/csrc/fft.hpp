@@ -21,7 +21,8 @@ cyr_status fft3d(void *const *vols, uint32_t count, uint32_t nx, uint32_t ny, ui
                  hipStream_t stream);
 
 // Twiddle table W_n[m] = exp(-2 pi i m / n), m < n, generated in FP64 on the
-// host and cached per device (stored in the compute precision).
+// host and cached per device (stored in the compute precision; FP32 tables add
+// the block of rounding residuals, tw_len<T>(n) entries in all).
 template <typename T> const cx<T> *twiddles(uint32_t n, cyr_status *st);
 
 // Launch-shape override for the tuner: elements per thread and pencils per block

@@ -42,8 +42,13 @@ template <typename T, int E> struct RtFft {
     return ((uint32_t)i / R0) * tpp + ((uint32_t)i % R0) * (P.n / R0);
   }
   __device__ bool exchanges() const { return P.nstages > 1; }
-  __device__ void run(cx<T> (&v)[E], uint32_t t, uint32_t p, cx<T> *lds, const cx<T> *__restrict__ tw) const {
-    stockham_from_regs<T, E>(v, P, t, tpp, p, lds, L, tw);
+  // tw: the length-n twiddle block (global or an LDS copy); twlo: its FP32
+  // residual block (may sit elsewhere; unused for FP64).  COMP = false keeps
+  // plain products in every stage (twlo unused).
+  template <bool COMP = true>
+  __device__ void run(cx<T> (&v)[E], uint32_t t, uint32_t p, cx<T> *lds, const cx<T> *__restrict__ tw,
+                      const cx<T> *twlo) const {
+    stockham_from_regs<T, E>(v, P, t, tpp, p, lds, L, tw, COMP ? twlo : nullptr);
   }
 };
 
@@ -107,22 +112,31 @@ __device__ inline void ct_load(cx<T> (&v)[E], uint32_t t, const cx<T> *lb) {
 // is the last) the LDS exchange into the next stage's input order.  lb is the
 // pencil's LDS base, PE the LDS stride between consecutive pencil elements
 // (PE = 1: padded row, see lds_slot).
-template <typename T, int N, int E, int PE, int LS, int R, int... Rest>
-__device__ inline void ct_stages(cx<T> (&v)[E], uint32_t t, cx<T> *lb, const cx<T> *__restrict__ tw) {
+template <typename T, int N, int E, int PE, bool CP, int LS, int R, int... Rest>
+__device__ inline void ct_stages(cx<T> (&v)[E], uint32_t t, cx<T> *lb, const cx<T> *__restrict__ tw,
+                                 const cx<T> *__restrict__ twlo) {
   constexpr int TPP = N / E, NB = E / R, TSTEP = N / (LS * R);
+  // with CP the last stage applies the finest twiddles as compensated products (FP32 hi + lo)
+  constexpr bool COMP = CP && sizeof...(Rest) == 0 && tw_blocks<T>() == 2;
   static_assert(E % R == 0, "elements per thread must be a multiple of every radix");
   if constexpr (LS > 1) {
     if constexpr (TPP % LS == 0) {
       // every butterfly of this thread has the same twiddle exponent k = t mod LS
       const uint32_t k = t % LS;
       if (k != 0) {
-        cx<T> w[R];
+        cx<T> w[R], wl[R];
 #pragma unroll
-        for (int r = 1; r < R; r++) w[r] = tw[r * k * TSTEP];
+        for (int r = 1; r < R; r++) {
+          w[r] = tw[r * k * TSTEP];
+          if constexpr (COMP) wl[r] = twlo[r * k * TSTEP];
+        }
 #pragma unroll
         for (int b = 0; b < NB; b++)
 #pragma unroll
-          for (int r = 1; r < R; r++) v[b * R + r] = cmul<T>(v[b * R + r], w[r]);
+          for (int r = 1; r < R; r++) {
+            if constexpr (COMP) v[b * R + r] = cmul_comp<T>(v[b * R + r], w[r], wl[r]);
+            else v[b * R + r] = cmul<T>(v[b * R + r], w[r]);
+          }
       }
     } else {
 #pragma unroll
@@ -130,7 +144,10 @@ __device__ inline void ct_stages(cx<T> (&v)[E], uint32_t t, cx<T> *lb, const cx<
         const uint32_t k = (t + b * TPP) % LS;
         if (k != 0) {
 #pragma unroll
-          for (int r = 1; r < R; r++) v[b * R + r] = cmul<T>(v[b * R + r], tw[r * k * TSTEP]);
+          for (int r = 1; r < R; r++) {
+            if constexpr (COMP) v[b * R + r] = cmul_comp<T>(v[b * R + r], tw[r * k * TSTEP], twlo[r * k * TSTEP]);
+            else v[b * R + r] = cmul<T>(v[b * R + r], tw[r * k * TSTEP]);
+          }
         }
       }
     }
@@ -155,7 +172,7 @@ __device__ inline void ct_stages(cx<T> (&v)[E], uint32_t t, cx<T> *lb, const cx<
     static_assert(E % R2 == 0, "elements per thread must be a multiple of every radix");
     ct_load<T, E, PE, TPP, N / R2, R2, 0>(v, t, lb);
     lds_sync();
-    ct_stages<T, N, E, PE, LS * R, Rest...>(v, t, lb, tw);
+    ct_stages<T, N, E, PE, CP, LS * R, Rest...>(v, t, lb, tw, twlo);
   }
 }
 
@@ -171,9 +188,11 @@ template <typename T, int N, int E, bool CONTIG, int PP, int... R> struct CtFft 
   __device__ static uint32_t out_pos(uint32_t t, int i) { return t + (i / RL) * TPP + (i % RL) * (N / RL); }
   __device__ static constexpr uint32_t in_delta(int i) { return (i / R0) * TPP + (i % R0) * (N / R0); }
   __device__ static constexpr bool exchanges() { return sizeof...(R) > 1; }
-  __device__ static void run(cx<T> (&v)[E], uint32_t t, uint32_t p, cx<T> *lds, const cx<T> *__restrict__ tw) {
+  template <bool COMP = true>
+  __device__ static void run(cx<T> (&v)[E], uint32_t t, uint32_t p, cx<T> *lds, const cx<T> *__restrict__ tw,
+                             const cx<T> *__restrict__ twlo) {
     cx<T> *lb = CONTIG ? lds + (size_t)p * kPitch : lds + p;
-    ct_stages<T, N, E, PE, 1, R...>(v, t, lb, tw);
+    ct_stages<T, N, E, PE, COMP, 1, R...>(v, t, lb, tw, twlo);
   }
 };
 

