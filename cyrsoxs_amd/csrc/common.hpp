@@ -6,6 +6,7 @@
 #include <hip/hip_bf16.h>
 
 #include <cstdint>
+#include <type_traits>
 #include <string>
 #include <vector>
 
@@ -103,6 +104,15 @@ template <typename T> __device__ inline double tw_cos(const cx<T> *tw, uint32_t 
 template <typename T> __device__ inline cx<T> cmul_tw(cx<T> v, const cx<T> *tw, uint32_t n, uint32_t e) {
   if constexpr (sizeof(T) == 4) return cmul_comp<T>(v, tw[e], tw[n + e]);
   else return cmul<T>(v, tw[e]);
+}
+// Compile-time loop: fn(std::integral_constant<int, I>) for I in [I0, N).  Used
+// where a #pragma unroll may be declined (large bodies), which would turn
+// register arrays indexed by the loop variable into scratch memory.
+template <int I, int N, class Fn> __device__ inline void static_for(Fn &&fn) {
+  if constexpr (I < N) {
+    fn(std::integral_constant<int, I>{});
+    static_for<I + 1, N>(fn);
+  }
 }
 // multiply by -i
 template <typename T> __device__ inline cx<T> cmul_mi(cx<T> a) { return mk<T>(a.y, -a.x); }

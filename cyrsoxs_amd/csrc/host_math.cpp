@@ -221,6 +221,12 @@ cyr_status build_geometry(const cyr_sim_desc *d, SimGeometry &g) {
   g.P = (uint64_t)g.nx * g.ny;
   const double inc = d->angle_increment == 0 ? 1.0 : d->angle_increment;
   g.nA = num_angles(d->angle_start, inc, d->angle_end);
+  g.quarter = 0;
+  if (inc > 0 && inc <= 90.0) {
+    const double steps = std::round(90.0 / inc);
+    if (std::fabs(steps * inc - 90.0) < 1e-9 && g.nA % (uint32_t)steps == 0 && g.nA / (uint32_t)steps >= 2)
+      g.quarter = (uint32_t)steps;
+  }
   std::vector<V3> ks;
   if (d->case_type == 0 && d->num_kvectors == 0) ks.push_back(V3{0, 0, 1});
   for (uint32_t i = 0; i < d->num_kvectors; i++)

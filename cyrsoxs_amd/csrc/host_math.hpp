@@ -43,6 +43,9 @@ uint32_t num_angles(double start, double inc, double end);
 // Everything the executors need that is fixed for one simulation.
 struct SimGeometry {
   uint32_t nx = 0, ny = 0, nz = 0, M = 0, nE = 0, nK = 0, nA = 0;
+  // angle steps per 90 degrees when the E-angle list steps an exact divisor of
+  // 90 degrees and spans whole quarter turns (0 otherwise)
+  uint32_t quarter = 0;
   uint64_t V = 0, P = 0;
   std::vector<M3> RK;          // per k
   std::vector<double> base;    // per k base angle (radians)

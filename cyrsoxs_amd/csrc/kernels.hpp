@@ -204,13 +204,22 @@ void rotation_tile_order(uint32_t nx, uint32_t ny, uint32_t *out);
 template <typename T>
 cyr_status launch_rotate_all(const T *proj, const double *inv_tab, uint32_t nA, uint32_t nx, uint32_t ny,
                              int rot_mask, double *avg, const uint32_t *tiles, hipStream_t st);
+// Quarter-turn orbits of a square N x N plane (N even): orbit representatives
+// in 16 x 16 tiles (ty << 16 | tx) ordered by distance from the rotation centre.
+uint32_t rotation_orbit_tile_count(uint32_t n);
+void rotation_orbit_tile_order(uint32_t n, uint32_t *out);
+// Quarter turns Q (2..4) that the orbit kernel can share taps over, given
+// `quarter` angle steps per 90 degrees (0: none); 0 when it does not apply.
+uint32_t rotation_quarter_turns(uint32_t nx, uint32_t ny, uint32_t nA, uint32_t quarter);
 // Laboratory frame: the same for ne energies from their per-pixel quadratic forms
 // quad[ne][Ny][Nx][4] (I(theta) = c^2 q0 + s^2 q1 + 2 c s q2) with coef[a] =
 // (c^2, s^2, 2 c s) into avg[ne][Ny][Nx]; each angle's taps serve several energies.
+// quarter (angle steps per 90 degrees, 0 = none) with otiles (orbit tile order)
+// lets each tap cell serve the angles a quarter turn apart.
 template <typename T>
 cyr_status launch_rotate_quad(const T *quad, const double *inv_tab, const double *coef, uint32_t nA, uint32_t nx,
                               uint32_t ny, int rot_mask, double *avg, uint32_t ne, const uint32_t *tiles,
-                              hipStream_t st);
+                              uint32_t quarter, const uint32_t *otiles, hipStream_t st);
 // Detector warp of the averaged pattern into the output (prefill 0 or NaN).
 template <typename T>
 cyr_status launch_detector_warp(const double *avg, T *out, uint32_t nx, uint32_t ny, const double inv[6],

@@ -93,9 +93,14 @@ template <typename T> MatConst<T> mat_const(const double *m4) {
   return r;
 }
 
+// Orbit tiles of the quarter-turn rotation kernel (0 when it does not apply).
+inline uint32_t orbit_tiles(const SimGeometry &g) {
+  return rotation_quarter_turns(g.nx, g.ny, g.nA, g.quarter) ? rotation_orbit_tile_count(g.nx) : 0;
+}
+
 // Per-simulation constant tables, uploaded once.
 struct Tables {
-  size_t mats = 0, han = 0, amat = 0, rotinv = 0, detinv = 0, kmag = 0, quad = 0, tiles = 0, total = 0;
+  size_t mats = 0, han = 0, amat = 0, rotinv = 0, detinv = 0, kmag = 0, quad = 0, tiles = 0, otiles = 0, total = 0;
   std::vector<unsigned char> host;
 };
 
@@ -162,6 +167,8 @@ template <typename T> void build_layout(const SimGeometry &g, Tables &tb) {
   off = align256(off + sizeof(double) * 3 * g.nK * g.nA);
   tb.tiles = off;
   off = align256(off + sizeof(uint32_t) * rotation_tile_count(g.nx, g.ny));
+  tb.otiles = off;
+  off = align256(off + sizeof(uint32_t) * orbit_tiles(g));
   tb.total = off;
 }
 
@@ -197,6 +204,7 @@ void build_tables(const cyr_sim_desc *d, const SimGeometry &g, const double *mat
     qc[3 * i + 2] = 2.0 * c * sn;
   }
   rotation_tile_order(g.nx, g.ny, reinterpret_cast<uint32_t *>(tb.host.data() + tb.tiles));
+  if (orbit_tiles(g)) rotation_orbit_tile_order(g.nx, reinterpret_cast<uint32_t *>(tb.host.data() + tb.otiles));
 }
 
 template <typename T> Vec3<T> v3(const V3 &v) { return Vec3<T>{(T)v.x, (T)v.y, (T)v.z}; }
@@ -211,7 +219,7 @@ template <typename T> struct Setup {
   const MatConst<T> *mats;
   const T *hx, *hy, *hz, *amat;
   const double *rotinv, *kmag, *quad;
-  const uint32_t *tiles;
+  const uint32_t *tiles, *otiles;  // otiles: null without quarter-turn orbits
   const cx<T> *twy, *twz;
   double vox_bytes, cxb;
 };
@@ -233,6 +241,7 @@ cyr_status setup_views(const cyr_sim_desc *d, const SimGeometry &g, const Tables
   su.kmag = reinterpret_cast<const double *>(tbase + tb.kmag);
   su.quad = reinterpret_cast<const double *>(tbase + tb.quad);
   su.tiles = reinterpret_cast<const uint32_t *>(tbase + tb.tiles);
+  su.otiles = orbit_tiles(g) ? reinterpret_cast<const uint32_t *>(tbase + tb.otiles) : nullptr;
   cyr_status s;
   su.twz = g.nz > 1 ? twiddles<T>(g.nz, &s) : nullptr;
   if (g.nz > 1 && !su.twz) return s;
@@ -653,7 +662,7 @@ cyr_status run_mbasis(const cyr_sim_desc *d, const SimGeometry &g, const void *d
           kt_end("mbasis_quad", s, 2.0 * nvol * (double)J * P * cxb + n * 4.0 * P * sizeof(T));
           kt_begin(s);
           CYR_TRY(launch_rotate_quad<T>(proj, inv, su.quad + (size_t)3 * k * g.nA, g.nA, g.nx, g.ny, d->rot_mask, avg,
-                                        n, su.tiles, s));
+                                        n, su.tiles, g.quarter, su.otiles, s));
           kt_end("rotate_quad", s, n * (4.0 * P * sizeof(T) + 8.0 * P));
           CYR_TRY(launch_detector_warp_n<T>(avg, d_out + ((uint64_t)b0 * g.nK + k) * P, (uint64_t)g.nK * P, n, g.nx,
                                             g.ny, &g.det_inv[(size_t)k * 6], d->rot_mask, s));

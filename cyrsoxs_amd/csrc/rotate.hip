@@ -110,8 +110,109 @@ __global__ void __launch_bounds__(256) rotate_all_kernel(const T *__restrict__ p
 // q = (q0, q1, q2): I(theta) = c^2 q0 + s^2 q1 + 2 c s q2.  Bilinear sampling is
 // linear, so the three components are sampled (same taps, weights and validity
 // as bilinear_sample; a tap of zero weight contributes nothing, NaN included)
-// and combined once per angle, in the storage precision (FP64 runs: FP64).  Each angle's back-mapped point, taps and
-// weights serve NE energies (grid y = group of NE energies, ne in total).
+// and combined once per angle, in the storage precision (FP64 runs: FP64).  Each
+// angle's back-mapped point, taps and weights serve NE energies.
+
+// Back-mapped point of one (pixel, angle): validity, top-left tap, fractions.
+struct QPoint {
+  uint32_t x0, y0;
+  double fx, fy;
+  bool inside;
+};
+
+__device__ inline QPoint quad_point(const double *inv, uint32_t X, uint32_t Y, uint32_t nx, uint32_t ny) {
+  double sx, sy;
+  back_map_point(inv, X, Y, sx, sy);
+  QPoint p;
+  p.inside = sx >= 0 && sx <= (double)(nx - 1) && sy >= 0 && sy <= (double)(ny - 1);
+  const double fx0 = p.inside ? floor(sx) : 0.0, fy0 = p.inside ? floor(sy) : 0.0;
+  p.x0 = (uint32_t)fx0;
+  p.y0 = (uint32_t)fy0;
+  p.fx = p.inside ? sx - fx0 : 0.0;
+  p.fy = p.inside ? sy - fy0 : 0.0;
+  return p;
+}
+
+template <typename T> using Quad4 = typename std::conditional<sizeof(T) == 4, float4, double4>::type;
+
+// The four taps of p (clamped into the image) for NE energy planes.
+template <typename T, int NE>
+__device__ inline void quad_gather(const Quad4<T> *const (&q4)[NE], const QPoint &p, uint32_t nx, uint32_t ny,
+                                   Quad4<T> (&tp)[NE][4]) {
+  const uint32_t x1 = p.x0 + 1 < nx ? p.x0 + 1 : p.x0, y1 = p.y0 + 1 < ny ? p.y0 + 1 : p.y0;
+  const uint32_t off[4] = {p.y0 * nx + p.x0, p.y0 * nx + x1, y1 * nx + p.x0, y1 * nx + x1};  // P < 2^32
+#pragma unroll
+  for (int e = 0; e < NE; e++)
+#pragma unroll
+    for (int i = 0; i < 4; i++) tp[e][i] = q4[e][off[i]];
+}
+
+// Adds one angle's intensity (coefficients c = (c^2, s^2, 2cs)) at point p from
+// its taps tp to the FP64 sums of NE energies.
+template <typename T, int NE>
+__device__ inline void quad_add(const QPoint &p, const Quad4<T> (&tp)[NE][4], const double *c, int rot_mask,
+                                double (&acc)[NE], uint32_t (&cnt)[NE]) {
+  const double gx = 1 - p.fx, gy = 1 - p.fy;
+  const double wd[4] = {gx * gy, p.fx * gy, gx * p.fy, p.fx * p.fy};
+  // taps are interpolated in the storage precision (the weights rounded once per
+  // angle); which taps count (weight exactly 0 or not) is decided on the FP64
+  // weights, and the sum over angles stays FP64
+  T w[4];
+  bool z[4];
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    w[i] = (T)wd[i];
+    z[i] = wd[i] == 0;
+  }
+  const T c0 = (T)c[0], c1 = (T)c[1], c2 = (T)c[2];
+#pragma unroll
+  for (int e = 0; e < NE; e++) {
+    T s0 = 0, s1 = 0, s2 = 0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const Quad4<T> t = tp[e][i];
+      s0 += w[i] * (z[i] ? (T)0 : t.x);
+      s1 += w[i] * (z[i] ? (T)0 : t.y);
+      s2 += w[i] * (z[i] ? (T)0 : t.z);
+    }
+    const double v = p.inside ? (double)(c0 * s0 + c1 * s1 + c2 * s2) : __longlong_as_double(0x7ff8000000000000ll);
+    if (rot_mask) {
+      const bool ok = !isnan(v);
+      acc[e] += ok ? v : 0.0;
+      cnt[e] += ok ? 1u : 0u;
+    } else {
+      acc[e] += v;
+    }
+  }
+}
+
+template <int NE>
+__device__ inline void quad_store(double *avg, uint64_t P, uint64_t t, uint32_t e0, uint32_t ne, uint32_t nA,
+                                  int rot_mask, const double (&acc)[NE], const uint32_t (&cnt)[NE]) {
+#pragma unroll
+  for (int e = 0; e < NE; e++) {
+    if (e0 + e >= ne) break;
+    double *o = avg + (uint64_t)(e0 + e) * P;
+    if (rot_mask) o[t] = cnt[e] == 0 ? 0.0 : acc[e] * (1.0 / (cnt[e] * 1.0));
+    else o[t] = acc[e] * (1.0 / (double)nA);
+  }
+}
+
+// Every angle of one pixel, in angle order.
+template <typename T, int NE>
+__device__ inline void quad_pixel(const Quad4<T> *const (&q4)[NE], const double *__restrict__ inv_tab,
+                                  const double *__restrict__ coef, uint32_t nA, uint32_t nx, uint32_t ny, uint32_t X,
+                                  uint32_t Y, int rot_mask, double (&acc)[NE], uint32_t (&cnt)[NE]) {
+#pragma unroll 2
+  for (uint32_t a = 0; a < nA; a++) {
+    const QPoint p = quad_point(inv_tab + 6 * (uint64_t)a, X, Y, nx, ny);
+    Quad4<T> tp[NE][4];
+    quad_gather<T, NE>(q4, p, nx, ny, tp);
+    quad_add<T, NE>(p, tp, coef + 3 * (uint64_t)a, rot_mask, acc, cnt);
+  }
+}
+
+// grid y: groups of NE energies (ne in total)
 template <typename T, int NE>
 __global__ void __launch_bounds__(256) rotate_quad_kernel(const T *__restrict__ quad, const double *__restrict__ inv_tab,
                                                           const double *__restrict__ coef, uint32_t nA, uint32_t nx,
@@ -121,70 +222,91 @@ __global__ void __launch_bounds__(256) rotate_quad_kernel(const T *__restrict__ 
   const uint64_t P = (uint64_t)nx * ny;
   uint32_t X, Y;
   if (!tile_pixel(tiles, ntiles, nx, ny, X, Y)) return;
-  const uint64_t t = (uint64_t)Y * nx + X;
   const uint32_t e0 = blockIdx.z * NE;
-  using V4 = typename std::conditional<sizeof(T) == 4, float4, double4>::type;
-  const V4 *q4[NE];
+  const Quad4<T> *q4[NE];
 #pragma unroll
-  for (int e = 0; e < NE; e++) q4[e] = reinterpret_cast<const V4 *>(quad) + (uint64_t)(e0 + e < ne ? e0 + e : e0) * P;
-  double acc[NE];
-  uint32_t cnt[NE];
+  for (int e = 0; e < NE; e++) q4[e] = reinterpret_cast<const Quad4<T> *>(quad) + (uint64_t)(e0 + e < ne ? e0 + e : e0) * P;
+  double acc[NE] = {};
+  uint32_t cnt[NE] = {};
+  quad_pixel<T, NE>(q4, inv_tab, coef, nA, nx, ny, X, Y, rot_mask, acc, cnt);
+  quad_store<NE>(avg, P, (uint64_t)Y * nx + X, e0, ne, nA, rot_mask, acc, cnt);
+}
+
+// Quarter-turn orbits (square N x N planes whose angle list holds Q quarter turns
+// of A4 steps: angle a + q A4 is angle a plus 90q degrees).  The image rotation
+// turns about (N/2, N/2), so with s(X, Y) = (N - Y, X) (a 90-degree turn of the
+// pixel grid) the point pixel x samples at angle a + q A4 is the point pixel
+// s^q(x) samples at angle a.  One thread owns the orbit x, s x, s^2 x, s^3 x of a
+// pixel with X - N/2 in [1, N/2 - 1] and Y - N/2 in [0, N/2 - 1] (these orbits tile
+// [1, N-1]^2 minus the centre): it gathers the taps of each (s^i x, a < A4) once
+// and adds them to the Q (pixel, angle) pairs that sample there.  Each pair still
+// back-maps its own point with its own FP64 coefficients; when that point's tap
+// cell or validity differs from the shared one (rounding of the angle tables, at
+// most where a point lies on a pixel edge) it gathers its own taps, so every
+// pair's value is the one quad_pixel computes.  Only the order of the FP64 sums
+// over angles differs.  Row 0, column 0 and the centre (the tail blocks after the
+// orbit tiles) take every angle directly.
+template <typename T, int NE, int Q>
+__global__ void __launch_bounds__(256) rotate_orbit_kernel(const T *__restrict__ quad, const double *__restrict__ inv_tab,
+                                                           const double *__restrict__ coef, uint32_t nA, uint32_t A4,
+                                                           uint32_t n, int rot_mask, double *__restrict__ avg,
+                                                           uint32_t ne, const uint32_t *__restrict__ otiles,
+                                                           uint32_t n_otiles) {
+  const uint64_t P = (uint64_t)n * n;
+  const uint32_t h = n / 2;
+  const uint32_t qb = xcd_tile(blockIdx.x, gridDim.x);
+  const uint32_t e0 = blockIdx.z * NE;
+  const Quad4<T> *q4[NE];
 #pragma unroll
-  for (int e = 0; e < NE; e++) {
-    acc[e] = 0;
-    cnt[e] = 0;
+  for (int e = 0; e < NE; e++) q4[e] = reinterpret_cast<const Quad4<T> *>(quad) + (uint64_t)(e0 + e < ne ? e0 + e : e0) * P;
+  if (qb >= n_otiles) {
+    // tail: row 0 (n pixels), column 0 below it (n - 1) and the centre
+    const uint32_t j = (qb - n_otiles) * 256 + threadIdx.x;
+    if (j >= 2 * n) return;
+    const uint32_t X = j < n ? j : (j < 2 * n - 1 ? 0 : h);
+    const uint32_t Y = j < n ? 0 : (j < 2 * n - 1 ? j - n + 1 : h);
+    double acc[NE] = {};
+    uint32_t cnt[NE] = {};
+    quad_pixel<T, NE>(q4, inv_tab, coef, nA, n, n, X, Y, rot_mask, acc, cnt);
+    quad_store<NE>(avg, P, (uint64_t)Y * n + X, e0, ne, nA, rot_mask, acc, cnt);
+    return;
   }
-#pragma unroll 2
-  for (uint32_t a = 0; a < nA; a++) {
-    double sx, sy;
-    back_map_point(inv_tab + 6 * (uint64_t)a, X, Y, sx, sy);
-    const bool inside = sx >= 0 && sx <= (double)(nx - 1) && sy >= 0 && sy <= (double)(ny - 1);
-    const double fx0 = inside ? floor(sx) : 0.0, fy0 = inside ? floor(sy) : 0.0;
-    const uint32_t x0 = (uint32_t)fx0, y0 = (uint32_t)fy0;
-    const double fx = inside ? sx - fx0 : 0.0, fy = inside ? sy - fy0 : 0.0;
-    const uint32_t x1 = x0 + 1 < nx ? x0 + 1 : x0, y1 = y0 + 1 < ny ? y0 + 1 : y0;
-    const uint32_t off[4] = {y0 * nx + x0, y0 * nx + x1, y1 * nx + x0, y1 * nx + x1};  // P < 2^32
-    const double gx = 1 - fx, gy = 1 - fy;
-    const double wd[4] = {gx * gy, fx * gy, gx * fy, fx * fy};
-    // taps are interpolated in the storage precision (the weights rounded once per
-    // angle); which taps count (weight exactly 0 or not) is decided on the FP64
-    // weights, and the sum over angles stays FP64
-    T w[4];
-    bool z[4];
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-      w[i] = (T)wd[i];
-      z[i] = wd[i] == 0;
+  const uint32_t tt = otiles[qb];
+  const uint32_t w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const uint32_t u = (tt & 0xffffu) * 16 + (w & 1) * 8 + (l & 7) + 1;
+  const uint32_t v = (tt >> 16) * 16 + (w >> 1) * 8 + (l >> 3);
+  if (u >= h || v >= h) return;
+  // orbit pixels s^i(x), i = 0..3
+  const uint32_t X0 = h + u, Y0 = h + v;
+  const uint32_t PX[4] = {X0, n - Y0, n - X0, Y0};
+  const uint32_t PY[4] = {Y0, X0, n - Y0, n - X0};
+  double acc[4][NE] = {};
+  uint32_t cnt[4][NE] = {};
+  static_for<0, 4>([&](auto ic) {
+    constexpr int i = decltype(ic)::value;
+    for (uint32_t a = 0; a < A4; a++) {
+      const QPoint b = quad_point(inv_tab + 6 * (uint64_t)a, PX[i], PY[i], n, n);
+      Quad4<T> tp[NE][4];
+      quad_gather<T, NE>(q4, b, n, n, tp);
+      quad_add<T, NE>(b, tp, coef + 3 * (uint64_t)a, rot_mask, acc[i], cnt[i]);
+      static_for<1, Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        constexpr int j = (i - q + 4) & 3;  // s^q(pixel j) = pixel i
+        const uint32_t aa = a + q * A4;
+        const QPoint d = quad_point(inv_tab + 6 * (uint64_t)aa, PX[j], PY[j], n, n);
+        const bool own = d.inside && !(b.inside && d.x0 == b.x0 && d.y0 == b.y0);
+        if (own) {
+          Quad4<T> td[NE][4];
+          quad_gather<T, NE>(q4, d, n, n, td);
+          quad_add<T, NE>(d, td, coef + 3 * (uint64_t)aa, rot_mask, acc[j], cnt[j]);
+        } else {
+          quad_add<T, NE>(d, tp, coef + 3 * (uint64_t)aa, rot_mask, acc[j], cnt[j]);
+        }
+      });
     }
-    const double *c = coef + 3 * (uint64_t)a;
-    const T c0 = (T)c[0], c1 = (T)c[1], c2 = (T)c[2];
+  });
 #pragma unroll
-    for (int e = 0; e < NE; e++) {
-      T s0 = 0, s1 = 0, s2 = 0;
-#pragma unroll
-      for (int i = 0; i < 4; i++) {
-        const V4 tp = q4[e][off[i]];
-        s0 += w[i] * (z[i] ? (T)0 : tp.x);
-        s1 += w[i] * (z[i] ? (T)0 : tp.y);
-        s2 += w[i] * (z[i] ? (T)0 : tp.z);
-      }
-      const double v = inside ? (double)(c0 * s0 + c1 * s1 + c2 * s2) : __longlong_as_double(0x7ff8000000000000ll);
-      if (rot_mask) {
-        const bool ok = !isnan(v);
-        acc[e] += ok ? v : 0.0;
-        cnt[e] += ok ? 1u : 0u;
-      } else {
-        acc[e] += v;
-      }
-    }
-  }
-#pragma unroll
-  for (int e = 0; e < NE; e++) {
-    if (e0 + e >= ne) break;
-    double *o = avg + (uint64_t)(e0 + e) * P;
-    if (rot_mask) o[t] = cnt[e] == 0 ? 0.0 : acc[e] * (1.0 / (cnt[e] * 1.0));
-    else o[t] = acc[e] * (1.0 / (double)nA);
-  }
+  for (int i = 0; i < 4; i++) quad_store<NE>(avg, P, (uint64_t)PY[i] * n + PX[i], e0, ne, nA, rot_mask, acc[i], cnt[i]);
 }
 
 // grid y = pattern: avg + y P -> out + y out_stride
@@ -236,6 +358,29 @@ void rotation_tile_order(uint32_t nx, uint32_t ny, uint32_t *out) {
   for (size_t i = 0; i < r.size(); i++) out[i] = r[i].second;
 }
 
+uint32_t rotation_orbit_tile_count(uint32_t n) { return n >= 4 && n % 2 == 0 ? ceil_div(n / 2 - 1, 16) * ceil_div(n / 2, 16) : 0; }
+
+void rotation_orbit_tile_order(uint32_t n, uint32_t *out) {
+  // orbit representatives: u = X - n/2 in [1, n/2 - 1] (tile x covers u - 1),
+  // v = Y - n/2 in [0, n/2 - 1]; ordered by distance from the centre (0, 0)
+  const uint32_t h = n / 2, tx = ceil_div(h - 1, 16), ty = ceil_div(h, 16);
+  std::vector<std::pair<double, uint32_t>> r;
+  r.reserve((size_t)tx * ty);
+  for (uint32_t y = 0; y < ty; y++)
+    for (uint32_t x = 0; x < tx; x++) {
+      const double du = x * 16.0 + 9.0, dv = y * 16.0 + 8.0;
+      r.emplace_back(du * du + dv * dv, (y << 16) | x);
+    }
+  std::stable_sort(r.begin(), r.end(), [](const auto &a, const auto &b) { return a.first < b.first; });
+  for (size_t i = 0; i < r.size(); i++) out[i] = r[i].second;
+}
+
+uint32_t rotation_quarter_turns(uint32_t nx, uint32_t ny, uint32_t nA, uint32_t quarter) {
+  if (quarter == 0 || nx != ny || nx < 4 || nx % 2 != 0 || nA % quarter != 0) return 0;
+  const uint32_t q = nA / quarter;
+  return q >= 2 && q <= 4 ? q : 0;
+}
+
 template <typename T>
 cyr_status launch_rotate_accumulate(const T *proj, double *acc, uint32_t *count, uint32_t nx, uint32_t ny,
                                     const double inv[6], int rot_mask, hipStream_t st) {
@@ -264,13 +409,49 @@ cyr_status launch_rotate_all(const T *proj, const double *inv_tab, uint32_t nA, 
   return CYR_OK;
 }
 
+template <typename T, int NE>
+hipError_t launch_orbit(uint32_t Q, dim3 grid, hipStream_t st, const T *quad, const double *inv_tab,
+                        const double *coef, uint32_t nA, uint32_t A4, uint32_t n, int rot_mask, double *avg,
+                        uint32_t ne, const uint32_t *otiles, uint32_t n_otiles) {
+  switch (Q) {
+    case 2:
+      hipLaunchKernelGGL((rotate_orbit_kernel<T, NE, 2>), grid, dim3(256), 0, st, quad, inv_tab, coef, nA, A4, n,
+                         rot_mask, avg, ne, otiles, n_otiles);
+      break;
+    case 3:
+      hipLaunchKernelGGL((rotate_orbit_kernel<T, NE, 3>), grid, dim3(256), 0, st, quad, inv_tab, coef, nA, A4, n,
+                         rot_mask, avg, ne, otiles, n_otiles);
+      break;
+    default:
+      hipLaunchKernelGGL((rotate_orbit_kernel<T, NE, 4>), grid, dim3(256), 0, st, quad, inv_tab, coef, nA, A4, n,
+                         rot_mask, avg, ne, otiles, n_otiles);
+      break;
+  }
+  return hipGetLastError();
+}
+
 template <typename T>
 cyr_status launch_rotate_quad(const T *quad, const double *inv_tab, const double *coef, uint32_t nA, uint32_t nx,
                               uint32_t ny, int rot_mask, double *avg, uint32_t ne, const uint32_t *tiles,
-                              hipStream_t st) {
+                              uint32_t quarter, const uint32_t *otiles, hipStream_t st) {
   if (ne == 0) return CYR_OK;
   if ((uint64_t)nx * ny > 0xffffffffull) return fail(CYR_E_UNSUPPORTED, "rotation: more than 2^32 pixels");
   if (!tiles) return fail(CYR_E_INVALID, "rotation: tile order missing");
+  const uint32_t Q = otiles ? rotation_quarter_turns(nx, ny, nA, quarter) : 0;
+  if (Q) {
+    // quarter-turn orbits: each gathered tap cell serves Q (pixel, angle) pairs
+    const uint32_t n_otiles = rotation_orbit_tile_count(nx);
+    const uint64_t blocks = (uint64_t)n_otiles + ceil_div(2ull * nx, 256);
+    const bool big = (uint64_t)nx * ny * 4 * sizeof(T) > (16ull << 20);
+    const uint32_t NE = big ? 1 : 4;
+    const dim3 grid(xcd_grid(blocks), 1, ceil_div(ne, NE));
+    const hipError_t e = big ? launch_orbit<T, 1>(Q, grid, st, quad, inv_tab, coef, nA, quarter, nx, rot_mask, avg,
+                                                   ne, otiles, n_otiles)
+                             : launch_orbit<T, 4>(Q, grid, st, quad, inv_tab, coef, nA, quarter, nx, rot_mask, avg,
+                                                   ne, otiles, n_otiles);
+    if (e != hipSuccess) return hip_fail(e, "rotate_orbit_kernel", __FILE__, __LINE__);
+    return CYR_OK;
+  }
   const uint32_t ntiles = rotation_tile_count(nx, ny);
   // Energies per pass: 4 share each angle's taps; once a plane of quadratic forms
   // exceeds 16 MB (2048^2: 64 MB), one energy per pass keeps the gathers' working
@@ -320,7 +501,8 @@ cyr_status launch_warp(const T *src, T *dst, uint32_t nx, uint32_t ny, const dou
   template cyr_status launch_rotate_all<T>(const T *, const double *, uint32_t, uint32_t, uint32_t, int, double *, \
                                            const uint32_t *, hipStream_t);                                         \
   template cyr_status launch_rotate_quad<T>(const T *, const double *, const double *, uint32_t, uint32_t,        \
-                                            uint32_t, int, double *, uint32_t, const uint32_t *, hipStream_t);     \
+                                            uint32_t, int, double *, uint32_t, const uint32_t *, uint32_t,         \
+                                            const uint32_t *, hipStream_t);                                        \
   template cyr_status launch_detector_warp_n<T>(const double *, T *, uint64_t, uint32_t, uint32_t, uint32_t,       \
                                                 const double *, int, hipStream_t);                                 \
   template cyr_status launch_detector_warp<T>(const double *, T *, uint32_t, uint32_t, const double *, int,        \

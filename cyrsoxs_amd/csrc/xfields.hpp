@@ -40,15 +40,6 @@ enum XMode { kXTensor = 0, kXPair = 1, kXSingle = 2 };
 // Every mode runs 4-7 FFTs per row: the X twiddles are staged in LDS once per block.
 template <int MODE> constexpr bool x_lds_twiddles() { return true; }
 
-// Compile-time loop: fn(std::integral_constant<int, I>) for I in [I0, N).  Used
-// where a #pragma unroll may be declined (large bodies), which would turn
-// register arrays indexed by the loop variable into scratch memory.
-template <int I, int N, class Fn> __device__ inline void static_for(Fn &&fn) {
-  if constexpr (I < N) {
-    fn(std::integral_constant<int, I>{});
-    static_for<I + 1, N>(fn);
-  }
-}
 template <int MODE> constexpr int x_outputs() { return MODE == kXTensor ? kEbFields : (MODE == kXPair ? 7 : 4); }
 
 // ---- the five tensor fields of one voxel -------------------------------------

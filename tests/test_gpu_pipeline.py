@@ -93,6 +93,34 @@ def test_modes(case, prec, pipeline):
     print(f"  rel L2 = {err:.3e}")
 
 
+# Quarter-turn orbit rotation (laboratory frame of the material-basis executor):
+# angle lists spanning 2, 3 and 4 quarter turns, with exact multiples of 90
+# degrees (back-mapped points on pixel edges, where a pair gathers its own taps)
+# and without, rotation mask on and off.
+ORBIT_CASES = {
+    "q2_linear_hann": dict(dims=(48, 48, 8), angles=(0.0, 15.0, 165.0), kw=dict(interpolation=1, windowing=1)),
+    "q3_nn_rotmask": dict(dims=(40, 40, 8), angles=(0.0, 30.0, 240.0), kw=dict(interpolation=0, rot_mask=1)),
+    "q4_thin_linear": dict(dims=(64, 64, 1), angles=(0.0, 10.0, 350.0), kw=dict(interpolation=1)),
+    "q4_offset_rotmask": dict(dims=(32, 32, 8), angles=(7.0, 18.0, 349.0), kw=dict(interpolation=1, rot_mask=1)),
+}
+
+
+@pytest.mark.parametrize("prec", [capi.FP64, capi.FP32])
+@pytest.mark.parametrize("case", sorted(ORBIT_CASES))
+def test_quarter_turn_orbits(case, prec):
+    c = ORBIT_CASES[case]
+    print(f"orbit {case} prec={prec}", flush=True)
+    dims, M = c["dims"], 2
+    vox = random_euler(dims, M, 31)
+    energies = [283.0, 286.0, 289.0]
+    mats = random_constants(len(energies), M, 5)
+    phys = 1.0 if dims[2] == 1 else 5.0
+    ref = _oracle("orbit_" + case, O.Sim(dims, M, phys, energies, c["angles"], **c["kw"]), vox, mats)
+    sim = capi.Sim(dims, M, phys, energies, c["angles"], precision=prec, pipeline=capi.PIPE_MBASIS, **c["kw"])
+    err = _check(capi.simulate(sim, vox, mats), ref, prec)
+    print(f"  rel L2 = {err:.3e}")
+
+
 @pytest.mark.parametrize("prec", [capi.FP64, capi.FP32])
 def test_beam_divergence_kvectors(prec):
     """Case type 1: two tilted k vectors (the reference marks it experimental)."""
