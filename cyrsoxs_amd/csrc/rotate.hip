@@ -116,14 +116,14 @@ __global__ void __launch_bounds__(256) rotate_all_kernel(const T *__restrict__ p
 // Back-mapped point of one (pixel, angle): validity, top-left tap, fractions.
 struct QPoint {
   uint32_t x0, y0;
-  double fx, fy;
+  double sx, sy, fx, fy;
   bool inside;
 };
 
 __device__ inline QPoint quad_point(const double *inv, uint32_t X, uint32_t Y, uint32_t nx, uint32_t ny) {
-  double sx, sy;
-  back_map_point(inv, X, Y, sx, sy);
   QPoint p;
+  back_map_point(inv, X, Y, p.sx, p.sy);
+  const double sx = p.sx, sy = p.sy;
   p.inside = sx >= 0 && sx <= (double)(nx - 1) && sy >= 0 && sy <= (double)(ny - 1);
   const double fx0 = p.inside ? floor(sx) : 0.0, fy0 = p.inside ? floor(sy) : 0.0;
   p.x0 = (uint32_t)fx0;
@@ -147,11 +147,9 @@ __device__ inline void quad_gather(const Quad4<T> *const (&q4)[NE], const QPoint
     for (int i = 0; i < 4; i++) tp[e][i] = q4[e][off[i]];
 }
 
-// Adds one angle's intensity (coefficients c = (c^2, s^2, 2cs)) at point p from
-// its taps tp to the FP64 sums of NE energies.
+// Bilinear sums (q0, q1, q2) of NE energies at point p from its taps tp.
 template <typename T, int NE>
-__device__ inline void quad_add(const QPoint &p, const Quad4<T> (&tp)[NE][4], const double *c, int rot_mask,
-                                double (&acc)[NE], uint32_t (&cnt)[NE]) {
+__device__ inline void quad_sums(const QPoint &p, const Quad4<T> (&tp)[NE][4], T (&sm)[NE][3]) {
   const double gx = 1 - p.fx, gy = 1 - p.fy;
   const double wd[4] = {gx * gy, p.fx * gy, gx * p.fy, p.fx * p.fy};
   // taps are interpolated in the storage precision (the weights rounded once per
@@ -164,7 +162,6 @@ __device__ inline void quad_add(const QPoint &p, const Quad4<T> (&tp)[NE][4], co
     w[i] = (T)wd[i];
     z[i] = wd[i] == 0;
   }
-  const T c0 = (T)c[0], c1 = (T)c[1], c2 = (T)c[2];
 #pragma unroll
   for (int e = 0; e < NE; e++) {
     T s0 = 0, s1 = 0, s2 = 0;
@@ -175,7 +172,22 @@ __device__ inline void quad_add(const QPoint &p, const Quad4<T> (&tp)[NE][4], co
       s1 += w[i] * (z[i] ? (T)0 : t.y);
       s2 += w[i] * (z[i] ? (T)0 : t.z);
     }
-    const double v = p.inside ? (double)(c0 * s0 + c1 * s1 + c2 * s2) : __longlong_as_double(0x7ff8000000000000ll);
+    sm[e][0] = s0;
+    sm[e][1] = s1;
+    sm[e][2] = s2;
+  }
+}
+
+// Adds one angle's intensity (coefficients c = (c^2, s^2, 2cs)) from the sums sm
+// (NaN where the point is outside) to the FP64 sums of NE energies.
+template <typename T, int NE>
+__device__ inline void quad_combine(bool inside, const T (&sm)[NE][3], const double *c, int rot_mask,
+                                    double (&acc)[NE], uint32_t (&cnt)[NE]) {
+  const T c0 = (T)c[0], c1 = (T)c[1], c2 = (T)c[2];
+#pragma unroll
+  for (int e = 0; e < NE; e++) {
+    const double v =
+        inside ? (double)(c0 * sm[e][0] + c1 * sm[e][1] + c2 * sm[e][2]) : __longlong_as_double(0x7ff8000000000000ll);
     if (rot_mask) {
       const bool ok = !isnan(v);
       acc[e] += ok ? v : 0.0;
@@ -184,6 +196,14 @@ __device__ inline void quad_add(const QPoint &p, const Quad4<T> (&tp)[NE][4], co
       acc[e] += v;
     }
   }
+}
+
+template <typename T, int NE>
+__device__ inline void quad_add(const QPoint &p, const Quad4<T> (&tp)[NE][4], const double *c, int rot_mask,
+                                double (&acc)[NE], uint32_t (&cnt)[NE]) {
+  T sm[NE][3];
+  quad_sums<T, NE>(p, tp, sm);
+  quad_combine<T, NE>(p.inside, sm, c, rot_mask, acc, cnt);
 }
 
 template <int NE>
@@ -238,14 +258,22 @@ __global__ void __launch_bounds__(256) rotate_quad_kernel(const T *__restrict__ 
 // pixel grid) the point pixel x samples at angle a + q A4 is the point pixel
 // s^q(x) samples at angle a.  One thread owns the orbit x, s x, s^2 x, s^3 x of a
 // pixel with X - N/2 in [1, N/2 - 1] and Y - N/2 in [0, N/2 - 1] (these orbits tile
-// [1, N-1]^2 minus the centre): it gathers the taps of each (s^i x, a < A4) once
-// and adds them to the Q (pixel, angle) pairs that sample there.  Each pair still
-// back-maps its own point with its own FP64 coefficients; when that point's tap
-// cell or validity differs from the shared one (rounding of the angle tables, at
-// most where a point lies on a pixel edge) it gathers its own taps, so every
-// pair's value is the one quad_pixel computes.  Only the order of the FP64 sums
-// over angles differs.  Row 0, column 0 and the centre (the tail blocks after the
-// orbit tiles) take every angle directly.
+// [1, N-1]^2 minus the centre): it back-maps and gathers each (s^i x, a < A4) once
+// and serves the Q (pixel, angle) pairs that sample there.
+//
+// The pairs' own FP64 back-maps differ from the shared point by rounding of the
+// angle tables (~N 1e-16).  When the shared point lies farther than kOrbitMargin
+// from every cell edge and from the image border, all Q pairs have its tap cell,
+// validity and nonzero weights, so they reuse its bilinear sums (only the
+// weights' last bits can differ from a pair's own); otherwise (a point on or
+// near a pixel edge: angles at exact multiples of 90 degrees) each pair
+// back-maps its own point and gathers its own taps when its cell differs, as
+// quad_pixel does.  Validity and NaN taps are therefore decided exactly as per
+// pair; only the order of the FP64 sums over angles differs.  Row 0, column 0
+// and the centre (the tail blocks after the orbit tiles) take every angle
+// directly.
+constexpr double kOrbitMargin = 1e-9;
+
 template <typename T, int NE, int Q>
 __global__ void __launch_bounds__(256) rotate_orbit_kernel(const T *__restrict__ quad, const double *__restrict__ inv_tab,
                                                            const double *__restrict__ coef, uint32_t nA, uint32_t A4,
@@ -280,29 +308,42 @@ __global__ void __launch_bounds__(256) rotate_orbit_kernel(const T *__restrict__
   const uint32_t X0 = h + u, Y0 = h + v;
   const uint32_t PX[4] = {X0, n - Y0, n - X0, Y0};
   const uint32_t PY[4] = {Y0, X0, n - Y0, n - X0};
+  const double hi = (double)(n - 1);
   double acc[4][NE] = {};
   uint32_t cnt[4][NE] = {};
   static_for<0, 4>([&](auto ic) {
     constexpr int i = decltype(ic)::value;
     for (uint32_t a = 0; a < A4; a++) {
       const QPoint b = quad_point(inv_tab + 6 * (uint64_t)a, PX[i], PY[i], n, n);
+      const double m = kOrbitMargin;
+      const bool safe = b.inside ? (b.fx >= m && b.fx <= 1 - m && b.fy >= m && b.fy <= 1 - m)
+                                 : (b.sx < -m || b.sx > hi + m || b.sy < -m || b.sy > hi + m);
       Quad4<T> tp[NE][4];
       quad_gather<T, NE>(q4, b, n, n, tp);
-      quad_add<T, NE>(b, tp, coef + 3 * (uint64_t)a, rot_mask, acc[i], cnt[i]);
-      static_for<1, Q>([&](auto qc) {
-        constexpr int q = decltype(qc)::value;
-        constexpr int j = (i - q + 4) & 3;  // s^q(pixel j) = pixel i
-        const uint32_t aa = a + q * A4;
-        const QPoint d = quad_point(inv_tab + 6 * (uint64_t)aa, PX[j], PY[j], n, n);
-        const bool own = d.inside && !(b.inside && d.x0 == b.x0 && d.y0 == b.y0);
-        if (own) {
-          Quad4<T> td[NE][4];
-          quad_gather<T, NE>(q4, d, n, n, td);
-          quad_add<T, NE>(d, td, coef + 3 * (uint64_t)aa, rot_mask, acc[j], cnt[j]);
-        } else {
-          quad_add<T, NE>(d, tp, coef + 3 * (uint64_t)aa, rot_mask, acc[j], cnt[j]);
-        }
-      });
+      if (safe) {
+        T sm[NE][3];
+        quad_sums<T, NE>(b, tp, sm);
+        static_for<0, Q>([&](auto qc) {
+          constexpr int q = decltype(qc)::value;
+          constexpr int j = (i - q + 4) & 3;  // s^q(pixel j) = pixel i
+          quad_combine<T, NE>(b.inside, sm, coef + 3 * (uint64_t)(a + q * A4), rot_mask, acc[j], cnt[j]);
+        });
+      } else {
+        quad_add<T, NE>(b, tp, coef + 3 * (uint64_t)a, rot_mask, acc[i], cnt[i]);
+        static_for<1, Q>([&](auto qc) {
+          constexpr int q = decltype(qc)::value;
+          constexpr int j = (i - q + 4) & 3;
+          const uint32_t aa = a + q * A4;
+          const QPoint d = quad_point(inv_tab + 6 * (uint64_t)aa, PX[j], PY[j], n, n);
+          if (d.inside && !(b.inside && d.x0 == b.x0 && d.y0 == b.y0)) {
+            Quad4<T> td[NE][4];
+            quad_gather<T, NE>(q4, d, n, n, td);
+            quad_add<T, NE>(d, td, coef + 3 * (uint64_t)aa, rot_mask, acc[j], cnt[j]);
+          } else {
+            quad_add<T, NE>(d, tp, coef + 3 * (uint64_t)aa, rot_mask, acc[j], cnt[j]);
+          }
+        });
+      }
     }
   });
 #pragma unroll
@@ -441,14 +482,12 @@ cyr_status launch_rotate_quad(const T *quad, const double *inv_tab, const double
   if (Q) {
     // quarter-turn orbits: each gathered tap cell serves Q (pixel, angle) pairs
     const uint32_t n_otiles = rotation_orbit_tile_count(nx);
+    // 2 energies per pass (measured against 1 and 4: 512^2 x 20 energies 415 / 384 us
+    // for 4 / 2; 2048^2 film 7.30 / 6.75 / 8.92 ms per 10 energies for 1 / 2 / 4)
     const uint64_t blocks = (uint64_t)n_otiles + ceil_div(2ull * nx, 256);
-    const bool big = (uint64_t)nx * ny * 4 * sizeof(T) > (16ull << 20);
-    const uint32_t NE = big ? 1 : 4;
-    const dim3 grid(xcd_grid(blocks), 1, ceil_div(ne, NE));
-    const hipError_t e = big ? launch_orbit<T, 1>(Q, grid, st, quad, inv_tab, coef, nA, quarter, nx, rot_mask, avg,
-                                                   ne, otiles, n_otiles)
-                             : launch_orbit<T, 4>(Q, grid, st, quad, inv_tab, coef, nA, quarter, nx, rot_mask, avg,
-                                                   ne, otiles, n_otiles);
+    const dim3 grid(xcd_grid(blocks), 1, ceil_div(ne, 2));
+    const hipError_t e =
+        launch_orbit<T, 2>(Q, grid, st, quad, inv_tab, coef, nA, quarter, nx, rot_mask, avg, ne, otiles, n_otiles);
     if (e != hipSuccess) return hip_fail(e, "rotate_orbit_kernel", __FILE__, __LINE__);
     return CYR_OK;
   }
