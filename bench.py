@@ -33,11 +33,11 @@ HBM_PEAK_GBS = 8000.0
 # workload (material-basis executor), scaled by the counters' reading for a 1 GiB
 # negation (fetch x2.0, write x1.0); absent where not measured.
 TRAFFIC = {"mbasis": {"xfields": 2.954e9, "fft_y": 3.758e9, "mbasis_zband": 2.017e9, "mbasis_quad": 0.375e9,
-                      "rotate_quad": 1.087e9}}
+                      "rotate_quad": 0.274e9}}
 # timing labels of the library (cyr_kernel_timing) -> kernel symbol
 KERNEL_OF = {"xfields": "xfields_kernel", "fft_y": "fft_ct_kernel (Y pass)", "fft_z": "fft_ct_kernel (Z pass)",
              "mbasis_extract": "mbasis_extract_kernel", "mbasis_zband": "mbasis_zband_kernel", "mbasis_angles": "mbasis_angles_kernel",
-             "mbasis_quad": "mbasis_quad_kernel", "rotate_quad": "rotate_quad_kernel",
+             "mbasis_quad": "mbasis_quad_kernel", "rotate_quad": "rotate_orbit_kernel / rotate_quad_kernel",
              "mbasis_bands": "mbasis_bands_kernel", "ebasis_yz": "ebasis_yz_kernel", "ebasis_dc": "ebasis_dc_kernel",
              "ebasis_angles": "ebasis_angles_kernel", "ebasis_bins": "ebasis_bins_kernel",
              "rotate_all": "rotate_all_kernel"}
