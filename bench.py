@@ -99,6 +99,9 @@ def main(argv=None):
     ap.add_argument("--other-steps", type=int, default=2,
                     help="timed steps of the other fused executor (0 = skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--host-steps", type=int, default=1,
+                    help="steps of the host-buffer entry point (cyr_simulate: morphology uploaded and patterns "
+                         "returned over PCIe each call), reported beside the headline; 0 skips it")
     ap.add_argument("--energies", type=int, default=0,
                     help="run only the first N energies (profiling runs; 0 = all 20)")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
@@ -240,6 +243,24 @@ def main(argv=None):
                            "structure": "per angle: polarization, 3 full 3D FFTs, DC, Ewald, rotate/accumulate",
                            f"speedup_{a.pipeline}_vs_plain": value / (units * a.plain_steps / pel),
                            f"{a.pipeline}_vs_plain_max_rel_l2": rel, "nan_mask_identical": same}
+
+    if a.host_steps > 0:
+        # the reference's boundary hands over host buffers: the same simulation from a
+        # pageable host morphology ([M][V][4] float32, 2.15 GB) to host patterns
+        hv = vox.cpu().numpy()
+        log(f"host-buffer entry point: warmup 1, steps {a.host_steps}")
+        hout = capi.simulate(sim, hv, mats)
+        t0 = time.perf_counter()
+        for _ in range(a.host_steps):
+            hout = capi.simulate(sim, hv, mats)
+        hel = time.perf_counter() - t0
+        rel, same = compare(hout.astype(np.float64), out.double().cpu().numpy())
+        result["host_buffers"] = {"value": units * a.host_steps / hel, "ms_per_step": 1000.0 * hel / a.host_steps,
+                                  "steps": a.host_steps, "h2d_bytes": int(hv.nbytes),
+                                  "structure": "cyr_simulate: H2D of the morphology, the same device work, D2H of "
+                                               "the patterns (PCIe-inclusive; not the headline value)",
+                                  "max_rel_l2_vs_device_api": rel, "nan_mask_identical": same}
+        del hv, hout
 
     # ---- roofline of the dominant kernel (largest share of the timed region)
     tot = sum(k["total_us"] for k in kern.values())
