@@ -189,6 +189,12 @@ struct DeviceBuffer {
 };
 cyr_status ensure_buffer(DeviceBuffer &b, size_t bytes);
 void free_buffer(DeviceBuffer &b);
+// Host -> device copy of a caller's (pageable) buffer.  Large buffers go through
+// per-device pinned staging with several host threads and return once the copy
+// has landed (after the work already queued on st); small ones are one copy
+// queued on st.
+cyr_status upload_host(void *dst, const void *src, size_t n, hipStream_t st);
+void release_upload_staging();
 
 // ---------------------------------------------------------------------------
 // Opt-in per-kernel event timing (cyr_kernel_timing).  kt_begin/kt_end bracket

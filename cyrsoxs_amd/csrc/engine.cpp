@@ -373,10 +373,10 @@ cyr_status Engine::run_request(Slot *s, Request &r) {
   CYR_TRY(ensure_buffer(s->out, ob));
   std::vector<unsigned char> conv;
   if (r.desc.storage == CYR_STORE_NATIVE) {
-    CYR_HIP(hipMemcpyAsync(s->vox.ptr, r.vox, vb, hipMemcpyHostToDevice, s->st));
+    CYR_TRY(upload_host(s->vox.ptr, r.vox, vb, s->st));
   } else {
     convert_storage(&r.desc, r.vox, conv);
-    CYR_HIP(hipMemcpyAsync(s->vox.ptr, conv.data(), vb, hipMemcpyHostToDevice, s->st));
+    CYR_TRY(upload_host(s->vox.ptr, conv.data(), vb, s->st));
   }
   CYR_TRY(run_simulation(&r.desc, g, s->vox.ptr, r.mats.data(), s->out.ptr, s->wb, s->st, kRunAsync));
   CYR_HIP(hipMemcpyAsync(r.out, s->out.ptr, ob, hipMemcpyDeviceToHost, s->st));

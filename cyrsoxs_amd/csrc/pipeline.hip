@@ -1065,11 +1065,11 @@ cyr_status cyr_simulate(const cyr_sim_desc *desc, const void *voxels, const doub
   CYR_TRY(ensure_buffer(ds->vox, vb));
   CYR_TRY(ensure_buffer(ds->out, ob));
   if (desc->storage == CYR_STORE_NATIVE) {
-    CYR_HIP(hipMemcpyAsync(ds->vox.ptr, voxels, vb, hipMemcpyHostToDevice, ds->stream));
+    CYR_TRY(upload_host(ds->vox.ptr, voxels, vb, ds->stream));
   } else {
     std::vector<unsigned char> conv;
     convert_storage(desc, voxels, conv);
-    CYR_HIP(hipMemcpyAsync(ds->vox.ptr, conv.data(), vb, hipMemcpyHostToDevice, ds->stream));
+    CYR_TRY(upload_host(ds->vox.ptr, conv.data(), vb, ds->stream));
     CYR_HIP(hipStreamSynchronize(ds->stream));
   }
   CYR_TRY(run_simulation(desc, g, ds->vox.ptr, mats, ds->out.ptr, ds->wb, ds->stream));
@@ -1275,6 +1275,7 @@ cyr_status cyr_release_workspace(void) {
   hipSetDevice(cur);
   fft_release();
   release_batch_buffers();
+  release_upload_staging();
   return CYR_OK;
 }
 

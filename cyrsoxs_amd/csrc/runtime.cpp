@@ -1,12 +1,14 @@
 // Error plumbing, device buffers and small C-ABI utilities.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <atomic>
 #include <cstdio>
 #include <cstring>
 #include <map>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "common.hpp"
@@ -123,6 +125,111 @@ void kt_collect() {
     g_kt_pool.push_back(p.b);
   }
   g_kt_pending.clear();
+}
+
+// ---- host -> device upload of large pageable buffers ------------------------
+// kUpThreads host threads each take every kUpThreads-th kUpChunk piece: copy it
+// into one of their two pinned buffers, queue its DMA on their own stream, and
+// copy the next piece while that DMA runs.  Staging (pinned buffers, streams,
+// events) is created once per device and kept until cyr_release_workspace.
+namespace {
+constexpr int kUpThreads = 4;
+constexpr size_t kUpChunk = 32ull << 20;
+
+struct UploadStage {
+  void *pin[kUpThreads][2] = {};
+  hipStream_t st[kUpThreads] = {};
+  hipEvent_t ev[kUpThreads][2] = {};
+};
+std::mutex g_up_mu;
+std::map<int, UploadStage *> g_up;
+
+hipError_t make_stage(UploadStage &u) {
+  for (int t = 0; t < kUpThreads; t++) {
+    hipError_t e = hipStreamCreateWithFlags(&u.st[t], hipStreamNonBlocking);  // default priority
+    for (int b = 0; b < 2 && e == hipSuccess; b++) {
+      e = hipHostMalloc(&u.pin[t][b], kUpChunk, hipHostMallocDefault);
+      if (e == hipSuccess) e = hipEventCreateWithFlags(&u.ev[t][b], hipEventDisableTiming);
+    }
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+void free_stage(UploadStage &u) {
+  for (int t = 0; t < kUpThreads; t++) {
+    if (u.st[t]) (void)hipStreamSynchronize(u.st[t]);
+    for (int b = 0; b < 2; b++) {
+      if (u.pin[t][b]) (void)hipHostFree(u.pin[t][b]);
+      if (u.ev[t][b]) (void)hipEventDestroy(u.ev[t][b]);
+      u.pin[t][b] = nullptr;
+      u.ev[t][b] = nullptr;
+    }
+    if (u.st[t]) (void)hipStreamDestroy(u.st[t]);
+    u.st[t] = nullptr;
+  }
+}
+}  // namespace
+
+cyr_status upload_host(void *dst, const void *src, size_t n, hipStream_t st) {
+  if (n < 2 * kUpChunk) {  // small: one pageable copy, ordered on st
+    CYR_HIP(hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, st));
+    return CYR_OK;
+  }
+  int dev = 0;
+  CYR_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_up_mu);  // one staged upload per process at a time
+  UploadStage *&u = g_up[dev];
+  if (!u) {
+    u = new UploadStage();
+    const hipError_t e = make_stage(*u);
+    if (e != hipSuccess) {
+      free_stage(*u);
+      delete u;
+      u = nullptr;
+      return hip_fail(e, "pinned upload staging", __FILE__, __LINE__);
+    }
+  }
+  // work queued earlier on st (e.g. a kernel still reading dst) finishes first
+  CYR_HIP(hipStreamSynchronize(st));
+  const size_t pieces = (n + kUpChunk - 1) / kUpChunk;
+  hipError_t err[kUpThreads];
+  auto work = [&](int t) {
+    hipError_t e = hipSetDevice(dev);
+    size_t k = 0;
+    for (size_t c = (size_t)t; c < pieces && e == hipSuccess; c += kUpThreads, k++) {
+      const int b = (int)(k & 1);
+      if (k >= 2) e = hipEventSynchronize(u->ev[t][b]);  // that buffer's previous DMA is done
+      if (e != hipSuccess) break;
+      const size_t off = c * kUpChunk, len = std::min(kUpChunk, n - off);
+      std::memcpy(u->pin[t][b], static_cast<const unsigned char *>(src) + off, len);
+      e = hipMemcpyAsync(static_cast<unsigned char *>(dst) + off, u->pin[t][b], len, hipMemcpyHostToDevice, u->st[t]);
+      if (e == hipSuccess) e = hipEventRecord(u->ev[t][b], u->st[t]);
+    }
+    const hipError_t e2 = hipStreamSynchronize(u->st[t]);
+    err[t] = e != hipSuccess ? e : e2;
+  };
+  std::vector<std::thread> th;
+  for (int t = 1; t < kUpThreads; t++) th.emplace_back(work, t);
+  work(0);
+  for (auto &x : th) x.join();
+  for (int t = 0; t < kUpThreads; t++)
+    if (err[t] != hipSuccess) return hip_fail(err[t], "staged upload", __FILE__, __LINE__);
+  return CYR_OK;
+}
+
+void release_upload_staging() {
+  std::lock_guard<std::mutex> lk(g_up_mu);
+  int cur = 0;
+  (void)hipGetDevice(&cur);
+  for (auto &kv : g_up) {
+    if (!kv.second) continue;
+    (void)hipSetDevice(kv.first);
+    free_stage(*kv.second);
+    delete kv.second;
+  }
+  g_up.clear();
+  (void)hipSetDevice(cur);
 }
 
 }  // namespace cyr
