@@ -262,6 +262,25 @@ def main(argv=None):
                                   "max_rel_l2_vs_device_api": rel, "nan_mask_identical": same}
         del hv, hout
 
+    # achievable HBM bandwidth on this device: a 1 GiB device copy and a 1 GiB
+    # elementwise negation (each reads 1 GiB and writes 1 GiB), best of the two,
+    # timed with events on torch's stream (where they run)
+    n_copy = 1 << 28
+    src = torch.ones(n_copy, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    copy_gbs = 0.0
+    for fn in (lambda: dst.copy_(src), lambda: torch.neg(src, out=dst)):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(10):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        copy_gbs = max(copy_gbs, 2.0 * n_copy * 4 / (e0.elapsed_time(e1) / 10 * 1e-3) / 1e9)
+    del src, dst
+
     # ---- roofline of the dominant kernel (largest share of the timed region)
     tot = sum(k["total_us"] for k in kern.values())
     table = {}
@@ -274,6 +293,7 @@ def main(argv=None):
     result["roofline"] = {"bound": "hbm", "kernel": KERNEL_OF.get(dom, dom), "achieved": ach, "peak": HBM_PEAK_GBS,
                           "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": TRAFFIC.get(a.pipeline, {}).get(dom),
                           "algorithmic_bytes": kern[dom]["avg_bytes"], "avg_us": kern[dom]["avg_us"],
+                          "achievable_copy_GBs": round(copy_gbs, 1), "frac_of_copy": round(ach / copy_gbs, 4),
                           "kernels": table}
 
     if not a.no_cpu_baseline and rank == 0:
