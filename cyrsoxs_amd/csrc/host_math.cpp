@@ -225,7 +225,7 @@ cyr_status build_geometry(const cyr_sim_desc *d, SimGeometry &g) {
   if (inc > 0 && inc <= 90.0) {
     const double steps = std::round(90.0 / inc);
     // (a tight bound: the orbit rotation kernel assumes the tables' quarter turns
-    // agree to far below its 1e-9 pixel margin)
+    // agree to far below its pixel margin of 1e-9 + N 1e-12)
     if (std::fabs(steps * inc - 90.0) < 1e-12 && g.nA % (uint32_t)steps == 0 && g.nA / (uint32_t)steps >= 2)
       g.quarter = (uint32_t)steps;
   }

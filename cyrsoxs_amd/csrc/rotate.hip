@@ -262,8 +262,9 @@ __global__ void __launch_bounds__(256) rotate_quad_kernel(const T *__restrict__ 
 // and serves the Q (pixel, angle) pairs that sample there.
 //
 // The pairs' own FP64 back-maps differ from the shared point by rounding of the
-// angle tables (~N 1e-16).  When the shared point lies farther than kOrbitMargin
-// from every cell edge and from the image border, all Q pairs have its tap cell,
+// angle tables (~N 1e-15 pixels: at most 4e-12 over every pair of a 4096^2 plane
+// with 720 angles).  When the shared point lies farther than the margin
+// 1e-9 + N 1e-12 from every cell edge and from the image border, all Q pairs have its tap cell,
 // validity and nonzero weights, so they reuse its bilinear sums (only the
 // weights' last bits can differ from a pair's own); otherwise (a point on or
 // near a pixel edge: angles at exact multiples of 90 degrees) each pair
@@ -272,7 +273,6 @@ __global__ void __launch_bounds__(256) rotate_quad_kernel(const T *__restrict__ 
 // pair; only the order of the FP64 sums over angles differs.  Row 0, column 0
 // and the centre (the tail blocks after the orbit tiles) take every angle
 // directly.
-constexpr double kOrbitMargin = 1e-9;
 
 template <typename T, int NE, int Q>
 __global__ void __launch_bounds__(256) rotate_orbit_kernel(const T *__restrict__ quad, const double *__restrict__ inv_tab,
@@ -315,7 +315,7 @@ __global__ void __launch_bounds__(256) rotate_orbit_kernel(const T *__restrict__
     constexpr int i = decltype(ic)::value;
     for (uint32_t a = 0; a < A4; a++) {
       const QPoint b = quad_point(inv_tab + 6 * (uint64_t)a, PX[i], PY[i], n, n);
-      const double m = kOrbitMargin;
+      const double m = 1e-9 + 1e-12 * (double)n;
       const bool safe = b.inside ? (b.fx >= m && b.fx <= 1 - m && b.fy >= m && b.fy <= 1 - m)
                                  : (b.sx < -m || b.sx > hi + m || b.sy < -m || b.sy > hi + m);
       Quad4<T> tp[NE][4];

@@ -196,6 +196,88 @@ __global__ void __launch_bounds__(WG, (MODE == kXTensor && F::kE <= 8) ? 4 : 2)
     lds_sync();  // (single-wave blocks: an LDS wait only)
     twp = ltw;
   }
+  if constexpr (MODE == kXPair) {
+    // Material pair: a compact per-material state per element (window-weighted
+    // phi_a, the second scalar field, the director s'; 5-6 values instead of the
+    // 7 fields) and each output's fields formed just before its FFT, so the
+    // loads are consumed early and the field products interleave with the FFTs
+    // (512-point rows, 2 materials: 577 us against 625 us with all 7 x 2 fields
+    // formed up front).
+    auto pair_rows = [&](auto mtag) {
+      constexpr int MORPH = decltype(mtag)::value;
+      T wal[MORPH == 1 ? 2 : 1][E], wpa[2][E], wbe[2][E], sxa[2][E], sya[2][E], sza[2][E];
+#pragma unroll
+      for (int i = 0; i < E; i++) {
+        const uint32_t x = fft.in_pos(t, i);
+        const T w = active ? (hann ? a.hx[x] * wyz : wyz) : (T)0;
+#pragma unroll
+        for (int m = 0; m < 2; m++) {
+          T vv[4] = {0, 0, 0, 0};
+          if (active) VoxLoad<T, S>::get(vox, (uint64_t)(a.m0 + m) * a.V + (uint64_t)row * a.nx + x, vv);
+          T sx, sy, sz;
+          if constexpr (MORPH == 1) {  // alpha = |s|^2, beta = phi_u, phi_a = 1
+            sx = vv[0];
+            sy = vv[1];
+            sz = vv[2];
+            wal[m][i] = w * (sx * sx + sy * sy + sz * sz);
+            wpa[m][i] = w;
+            wbe[m][i] = w * vv[3];
+          } else {  // alpha = phi_a = Vfrac S, beta = Vfrac - phi_a
+            T st, ct, sp, cp;
+            sincos_t(vv[1], &st, &ct);
+            sincos_t(vv[2], &sp, &cp);
+            sx = cp * st;
+            sy = sp * st;
+            sz = ct;
+            const T pa = vv[3] * vv[0];
+            wpa[m][i] = w * pa;
+            wbe[m][i] = w * (vv[3] - pa);
+          }
+          if (rot) {  // s' = Q^T s
+            const T q0 = a.Q.m[0] * sx + a.Q.m[3] * sy + a.Q.m[6] * sz;
+            const T q1 = a.Q.m[1] * sx + a.Q.m[4] * sy + a.Q.m[7] * sz;
+            const T q2 = a.Q.m[2] * sx + a.Q.m[5] * sy + a.Q.m[8] * sz;
+            sx = q0;
+            sy = q1;
+            sz = q2;
+          }
+          sxa[m][i] = sx;
+          sya[m][i] = sy;
+          sza[m][i] = sz;
+        }
+      }
+      // outputs c = 0..6: (alpha, beta, q00, q11, q01, q02, q12) of material m0
+      // (real part) and m0 + 1 (imaginary part), q_ij = phi_a s'_i s'_j
+      static_for<0, NO>([&](auto cc) {
+        constexpr int c = decltype(cc)::value;
+        cx<T> v[E];
+#pragma unroll
+        for (int i = 0; i < E; i++) {
+          T f[2];
+#pragma unroll
+          for (int m = 0; m < 2; m++) {
+            if constexpr (c == 0) f[m] = MORPH == 1 ? wal[MORPH == 1 ? m : 0][i] : wpa[m][i];
+            else if constexpr (c == 1) f[m] = wbe[m][i];
+            else if constexpr (c == 2) f[m] = wpa[m][i] * sxa[m][i] * sxa[m][i];
+            else if constexpr (c == 3) f[m] = wpa[m][i] * sya[m][i] * sya[m][i];
+            else if constexpr (c == 4) f[m] = wpa[m][i] * sxa[m][i] * sya[m][i];
+            else if constexpr (c == 5) f[m] = wpa[m][i] * sxa[m][i] * sza[m][i];
+            else f[m] = wpa[m][i] * sya[m][i] * sza[m][i];
+          }
+          v[i] = mk<T>(f[0], f[1]);
+        }
+        fft.template run<false>(v, t, p, lds, twp, nullptr);
+        if (active) {
+          cx<T> *o = a.out[c] + (uint64_t)row * a.nx;
+#pragma unroll
+          for (int i = 0; i < E; i++) o[fft.out_pos(t, i)] = v[i];
+        }
+      });
+    };
+    if (morph == 1) pair_rows(std::integral_constant<int, 1>{});
+    else pair_rows(std::integral_constant<int, 0>{});
+    return;
+  }
   T fre[NO][E], fim[NO][E];
 #pragma unroll
   for (int i = 0; i < E; i++) {
