@@ -40,19 +40,36 @@ def _run(cmd):
     return r.stdout
 
 
-def _compile(src):
+def _headers_mtime():
+    """Newest modification time of the library's headers (csrc/*.hpp, include/*.h)."""
+    t = 0.0
+    for d, ext in ((CSRC, ".hpp"), (os.path.join(REPO, "include"), ".h")):
+        for f in os.listdir(d):
+            if f.endswith(ext):
+                t = max(t, os.path.getmtime(os.path.join(d, f)))
+    return t
+
+
+def _compile(src, force=True, hdr_mtime=0.0):
     os.makedirs(OBJDIR, exist_ok=True)
     obj = os.path.join(OBJDIR, os.path.basename(src) + ".o")
+    path = os.path.join(CSRC, src)
+    if not force and os.path.exists(obj):
+        t = os.path.getmtime(obj)
+        if t >= os.path.getmtime(path) and t >= hdr_mtime and t >= os.path.getmtime(__file__):
+            return obj
     lang = ["-x", "hip"] if src.endswith(".hip") else []
-    cmd = [HIPCC, *COMMON, *PER_FILE.get(os.path.basename(src), []), *lang, "-c",
-           os.path.join(CSRC, src), "-o", obj]
+    cmd = [HIPCC, *COMMON, *PER_FILE.get(os.path.basename(src), []), *lang, "-c", path, "-o", obj]
     _run(cmd)
     return obj
 
 
-def build_lib(jobs=8):
+def build_lib(jobs=8, force=True):
+    """Compiles every source (force) or only those older than their source, a
+    header or this script, then links lib/libcyrsoxs.so."""
+    hdr = _headers_mtime()
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(_compile, LIB_SOURCES))
+        objs = list(ex.map(lambda s: _compile(s, force, hdr), LIB_SOURCES))
     os.makedirs(LIBDIR, exist_ok=True)
     out = os.path.join(LIBDIR, "libcyrsoxs.so")
     tmp = out + ".tmp"
@@ -102,8 +119,8 @@ def build_pymodule(jobs=8):
     return out
 
 
-def build_all(jobs=8, only=None):
-    lib = build_lib(jobs)
+def build_all(jobs=8, only=None, force=True):
+    lib = build_lib(jobs, force)
     if only == "lib":
         return lib
     build_cli(jobs)
@@ -115,8 +132,10 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--jobs", type=int, default=min(8, os.cpu_count() or 4))
     ap.add_argument("--only", default=None)
+    ap.add_argument("--incremental", action="store_true",
+                    help="recompile only sources older than their source file or a header")
     a = ap.parse_args(argv)
-    print(build_all(a.jobs, a.only))
+    print(build_all(a.jobs, a.only, force=not a.incremental))
 
 
 if __name__ == "__main__":

@@ -212,13 +212,89 @@ def _ptr(a):
     return ctypes.c_void_p(a.ctypes.data)
 
 
+# The C side receives bare pointers and cannot check them: every buffer is
+# validated here (element count, element type, contiguity, device) before a
+# pointer crosses the boundary.
+
+def _bad(where, msg):
+    raise CyrError(1, where, msg)  # CYR_E_INVALID
+
+
+def _voxel_count(sim):
+    nx, ny, nz = sim.dims
+    return int(sim.desc.num_materials) * nx * ny * nz * 4
+
+
+def _pattern_count(sim):
+    nx, ny, _ = sim.dims
+    return len(sim.energies) * sim.num_k * ny * nx
+
+
+def _mats(sim, mats, where):
+    m = np.ascontiguousarray(mats, dtype=np.float64)
+    want = len(sim.energies) * int(sim.desc.num_materials) * 4
+    if m.size != want:
+        _bad(where, f"optical constants hold {m.size} values; [nE][M][4] = {want} expected")
+    return m
+
+
+def _host_voxels(sim, voxels, where):
+    vox = np.ascontiguousarray(voxels, dtype=sim.real_dtype)
+    if vox.size != _voxel_count(sim):
+        _bad(where, f"morphology holds {vox.size} values; [M][V][4] = {_voxel_count(sim)} expected")
+    return vox
+
+
+def _host_out(sim, out, where):
+    if out is None:
+        nx, ny, _ = sim.dims
+        return np.empty((len(sim.energies), sim.num_k, ny, nx), dtype=sim.real_dtype)
+    if not isinstance(out, np.ndarray) or out.dtype != sim.real_dtype or not out.flags.c_contiguous \
+            or not out.flags.writeable or out.size != _pattern_count(sim):
+        _bad(where, f"output must be a writeable C-contiguous {np.dtype(sim.real_dtype).name} array of "
+                    f"{_pattern_count(sim)} values")
+    return out
+
+
+def _storage_dtypes(sim):
+    import torch
+    st = int(sim.desc.storage)
+    if st == STORE_FP16:
+        return (torch.float16,)
+    if st == STORE_BF16:
+        return (torch.bfloat16,)
+    return (torch.float64,) if sim.desc.precision == FP64 else (torch.float32,)
+
+
+def _device_tensor(t, dtypes, count, what, where):
+    import torch
+    if not isinstance(t, torch.Tensor):
+        _bad(where, f"{what} must be a torch tensor on the GPU")
+    if t.dtype not in dtypes:
+        _bad(where, f"{what} has dtype {t.dtype}; {' or '.join(str(d) for d in dtypes)} expected")
+    if not t.is_contiguous():
+        _bad(where, f"{what} is not contiguous")
+    if t.numel() != count:
+        _bad(where, f"{what} holds {t.numel()} elements; {count} expected")
+
+
+def _device_args(sim, d_voxels, d_out, where):
+    import torch
+    _device_tensor(d_voxels, _storage_dtypes(sim), _voxel_count(sim), "morphology", where)
+    od = torch.float64 if sim.desc.precision == FP64 else torch.float32
+    _device_tensor(d_out, (od,), _pattern_count(sim), "output", where)
+    for t, what in ((d_voxels, "morphology"), (d_out, "output")):
+        if not t.is_cuda:
+            _bad(where, f"{what} is on {t.device}, not on a GPU")
+    if d_voxels.device != d_out.device:
+        _bad(where, f"morphology on {d_voxels.device} and output on {d_out.device}")
+
+
 def simulate(sim: Sim, voxels, mats):
     """Host API: voxels [M][V][4] (compute precision), mats [nE][M][4] -> patterns [nE][nK][Ny][Nx]."""
-    rd = sim.real_dtype
-    vox = np.ascontiguousarray(voxels, dtype=rd)
-    m = np.ascontiguousarray(mats, dtype=np.float64)
-    nx, ny, nz = sim.dims
-    out = np.empty((len(sim.energies), sim.num_k, ny, nx), dtype=rd)
+    vox = _host_voxels(sim, voxels, "cyr_simulate")
+    m = _mats(sim, mats, "cyr_simulate")
+    out = _host_out(sim, None, "cyr_simulate")
     check(lib().cyr_simulate(sim.ref, _ptr(vox), _ptr(m), _ptr(out)), "cyr_simulate")
     return out
 
@@ -232,9 +308,11 @@ def _stream_ptr(stream):
 
 def simulate_device(sim: Sim, d_voxels, mats, d_out, stream=None):
     """Device API over torch tensors (inputs resident in HBM)."""
-    m = np.ascontiguousarray(mats, dtype=np.float64)
+    m = _mats(sim, mats, "cyr_simulate_device")
+    _device_args(sim, d_voxels, d_out, "cyr_simulate_device")
     check(lib().cyr_simulate_device(sim.ref, ctypes.c_void_p(d_voxels.data_ptr()), _ptr(m),
                                     ctypes.c_void_p(d_out.data_ptr()), _stream_ptr(stream)), "cyr_simulate_device")
+
 
 
 class YzTuning(ctypes.Structure):
@@ -273,8 +351,8 @@ def get_tuning(dims, precision):
 def polarization(sim: Sim, voxels, mats, energy_id, eangle_rad):
     rd = sim.real_dtype
     cd = np.complex128 if rd == np.float64 else np.complex64
-    vox = np.ascontiguousarray(voxels, dtype=rd)
-    m = np.ascontiguousarray(mats, dtype=np.float64)
+    vox = _host_voxels(sim, voxels, "cyr_polarization")
+    m = _mats(sim, mats, "cyr_polarization")
     nx, ny, nz = sim.dims
     V = nx * ny * nz
     px, py, pz = (np.empty(V, dtype=cd) for _ in range(3))
@@ -317,10 +395,18 @@ def warp_affine_device(src, dst, coeffs, precision, stream=None):
           "cyr_warp_affine_device")
 
 
+def _batch_args(sims, d_voxels, mats, d_outs, where):
+    if not (len(sims) == len(d_voxels) == len(mats) == len(d_outs)) or not sims:
+        _bad(where, "sims, morphologies, constants and outputs must be non-empty lists of one length")
+    for s, v, o in zip(sims, d_voxels, d_outs):
+        _device_args(s, v, o, where)
+    return [_mats(s, m, where) for s, m in zip(sims, mats)]
+
+
 def simulate_batch(sims, d_voxels, mats, d_outs, stream=None):
     n = len(sims)
+    mats_c = _batch_args(sims, d_voxels, mats, d_outs, "cyr_simulate_batch")
     descs = (SimDesc * n)(*[s.desc for s in sims])
-    mats_c = [np.ascontiguousarray(m, dtype=np.float64) for m in mats]
     vp = (ctypes.c_void_p * n)(*[v.data_ptr() for v in d_voxels])
     mp = (ctypes.c_void_p * n)(*[m.ctypes.data for m in mats_c])
     op = (ctypes.c_void_p * n)(*[o.data_ptr() for o in d_outs])
@@ -333,13 +419,13 @@ class BatchPlan:
 
     def __init__(self, sims, d_voxels, mats, d_outs, lanes=0):
         n = len(sims)
+        self.handle = ctypes.c_void_p()
+        mats_c = _batch_args(sims, d_voxels, mats, d_outs, "cyr_batch_create")
         self._keep = (sims, d_voxels, d_outs)
         descs = (SimDesc * n)(*[s.desc for s in sims])
-        mats_c = [np.ascontiguousarray(m, dtype=np.float64) for m in mats]
         vp = (ctypes.c_void_p * n)(*[v.data_ptr() for v in d_voxels])
         mp = (ctypes.c_void_p * n)(*[m.ctypes.data for m in mats_c])
         op = (ctypes.c_void_p * n)(*[o.data_ptr() for o in d_outs])
-        self.handle = ctypes.c_void_p()
         check(lib().cyr_batch_create(n, descs, vp, mp, op, int(lanes), ctypes.byref(self.handle)), "cyr_batch_create")
 
     def launch(self, stream=None):
@@ -374,12 +460,9 @@ class Engine:
         self._live = {}
 
     def submit(self, sim: Sim, voxels, mats, out=None):
-        rd = sim.real_dtype
-        vox = np.ascontiguousarray(voxels, dtype=rd)
-        m = np.ascontiguousarray(mats, dtype=np.float64)
-        nx, ny, nz = sim.dims
-        if out is None:
-            out = np.empty((len(sim.energies), sim.num_k, ny, nx), dtype=rd)
+        vox = _host_voxels(sim, voxels, "cyr_engine_submit")
+        m = _mats(sim, mats, "cyr_engine_submit")
+        out = _host_out(sim, out, "cyr_engine_submit")
         rid = ctypes.c_uint64()
         check(lib().cyr_engine_submit(self.handle, sim.ref, _ptr(vox), _ptr(m), _ptr(out), ctypes.byref(rid)),
               "cyr_engine_submit")

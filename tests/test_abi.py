@@ -149,3 +149,88 @@ def test_no_device_reported_without_gpu():
     with pytest.raises(capi.CyrError) as e:
         capi.simulate(s, vox, mats)
     assert e.value.status == 7
+
+
+# ------------------------------------------------------------- argument checks
+# The C side receives bare pointers; the binding rejects buffers of the wrong
+# size, type, layout or device before any pointer crosses the boundary.
+
+torch = pytest.importorskip("torch")
+
+
+def _small():
+    return _sim(dims=(8, 8, 4), num_materials=2, energies=[284.0, 285.0])
+
+
+def _dev_bufs(sim, vdtype=torch.float32, odtype=torch.float32):
+    nx, ny, nz = sim.dims
+    vox = torch.zeros((2, nx * ny * nz, 4), dtype=vdtype)
+    out = torch.zeros((2, 1, ny, nx), dtype=odtype)
+    return vox, out
+
+
+MATS = np.tile(np.array([1 - 1e-3, 1e-3, 1 - 1e-3, 1e-3]), (2, 2, 1))
+
+
+@pytest.mark.parametrize("fn", ["simulate_device"])
+def test_device_api_rejects_bad_buffers(fn):
+    sim = _small()
+    call = getattr(capi, fn)
+    vox, out = _dev_bufs(sim)
+    cases = {
+        "numpy morphology": (vox.numpy(), out, "torch tensor"),
+        "short morphology": (torch.zeros((2, 8 * 8 * 4 - 1, 4)), out, "elements"),
+        "wrong morphology type": (vox.double(), out, "dtype"),
+        "strided morphology": (torch.zeros((2, 4, 8 * 8 * 4)).transpose(1, 2), out, "contiguous"),
+        "wrong output type": (vox, out.double(), "dtype"),
+        "short output": (vox, out[:1], "elements"),
+        "host tensors": (vox, out, "not on a GPU"),
+    }
+    for name, (v, o, msg) in cases.items():
+        with pytest.raises(capi.CyrError) as e:
+            call(sim, v, MATS, o)
+        assert e.value.status == 1 and msg in str(e.value), (name, str(e.value))
+    with pytest.raises(capi.CyrError) as e:
+        call(sim, vox, MATS[:1], out)  # one energy's constants for two energies
+    assert "optical constants" in str(e.value)
+
+
+def test_storage_type_must_match_the_morphology_tensor():
+    sim16 = _sim(dims=(8, 8, 4), num_materials=2, energies=[284.0, 285.0], storage=capi.STORE_FP16)
+    vox, out = _dev_bufs(sim16)
+    with pytest.raises(capi.CyrError) as e:
+        capi.simulate_device(sim16, vox, MATS, out)  # float32 tensor for FP16 storage
+    assert "float16" in str(e.value)
+    vox16, _ = _dev_bufs(sim16, vdtype=torch.float16)
+    with pytest.raises(capi.CyrError) as e:
+        capi.simulate_device(sim16, vox16, MATS, out)  # right type, but on the host
+    assert "not on a GPU" in str(e.value)
+    sim64 = _sim(dims=(8, 8, 4), num_materials=2, energies=[284.0, 285.0], precision=capi.FP64)
+    v64, o32 = _dev_bufs(sim64, vdtype=torch.float64)
+    with pytest.raises(capi.CyrError) as e:
+        capi.simulate_device(sim64, v64, MATS, o32)  # FP64 run writes float64 patterns
+    assert "output" in str(e.value) and "dtype" in str(e.value)
+
+
+def test_host_api_rejects_bad_sizes():
+    sim = _small()
+    with pytest.raises(capi.CyrError) as e:
+        capi.simulate(sim, np.zeros((2, 8 * 8 * 4 - 1, 4)), MATS)
+    assert "morphology holds" in str(e.value)
+    with pytest.raises(capi.CyrError) as e:
+        capi.simulate(sim, np.zeros((2, 8 * 8 * 4, 4)), MATS[:, :1])
+    assert "optical constants" in str(e.value)
+    with pytest.raises(capi.CyrError) as e:
+        capi.polarization(sim, np.zeros((1, 8 * 8 * 4, 4)), MATS, 0, 0.0)
+    assert "morphology holds" in str(e.value)
+
+
+def test_batch_rejects_mismatched_lists():
+    sim = _small()
+    vox, out = _dev_bufs(sim)
+    with pytest.raises(capi.CyrError):
+        capi.simulate_batch([sim, sim], [vox], [MATS, MATS], [out, out])
+    with pytest.raises(capi.CyrError) as e:
+        capi.simulate_batch([sim], [vox], [MATS], [out])
+    assert "not on a GPU" in str(e.value)
+
