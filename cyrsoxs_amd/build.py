@@ -23,7 +23,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("CYRSOXS_ARCH", "gfx950")
 
 LIB_SOURCES = ["runtime.cpp", "host_math.cpp", "fft.hip", "polarization.hip", "ewald.hip", "ebasis.hip",
-               "mbasis.hip", "rotate.hip", "pipeline.hip", "engine.cpp"]
+               "mbasis.hip", "rotate.hip", "pipeline.hip", "noalloc.hip", "engine.cpp"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",
           # packed FP32 instructions are excluded from all device code

@@ -72,6 +72,7 @@ _lib = None
 _SIGS = {
     "cyr_simulate": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "cyr_simulate_device": (ctypes.c_int, [ctypes.c_void_p] * 5),
+    "cyr_simulate_device_noalloc": (ctypes.c_int, [ctypes.c_void_p] * 5),
     "cyr_polarization": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
                                         ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "cyr_polarization_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
@@ -313,6 +314,19 @@ def simulate_device(sim: Sim, d_voxels, mats, d_out, stream=None):
     check(lib().cyr_simulate_device(sim.ref, ctypes.c_void_p(d_voxels.data_ptr()), _ptr(m),
                                     ctypes.c_void_p(d_out.data_ptr()), _stream_ptr(stream)), "cyr_simulate_device")
 
+
+def simulate_device_noalloc(sim: Sim, d_voxels, mats, d_out, stream=None):
+    """cyr_simulate_device_noalloc: the device API with every work buffer in the
+    library's static scratch array (no device allocation); simulations larger
+    than NOALLOC_SCRATCH_BYTES are rejected with CYR_E_INVALID."""
+    m = _mats(sim, mats, "cyr_simulate_device_noalloc")
+    _device_args(sim, d_voxels, d_out, "cyr_simulate_device_noalloc")
+    check(lib().cyr_simulate_device_noalloc(sim.ref, ctypes.c_void_p(d_voxels.data_ptr()), _ptr(m),
+                                            ctypes.c_void_p(d_out.data_ptr()), _stream_ptr(stream)),
+          "cyr_simulate_device_noalloc")
+
+
+NOALLOC_SCRATCH_BYTES = 128 * 1024 * 1024  # CYR_NOALLOC_SCRATCH_BYTES
 
 
 class YzTuning(ctypes.Structure):

@@ -6,8 +6,10 @@
 #include <hip/hip_bf16.h>
 
 #include <cstdint>
-#include <type_traits>
+#include <map>
 #include <string>
+#include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "../../include/cyrsoxs.h"
@@ -189,6 +191,29 @@ struct DeviceBuffer {
 };
 cyr_status ensure_buffer(DeviceBuffer &b, size_t bytes);
 void free_buffer(DeviceBuffer &b);
+
+// A caller-provided device region that replaces allocation on the calling thread
+// while it is installed (the allocation-free entry point): ensure_buffer carves
+// 256-byte-aligned slices from it with a bump pointer (growing the topmost slice
+// in place) and fails with CYR_E_NOMEM instead of allocating; free_buffer forgets
+// slices without freeing them; twiddle tables are built into it per call, their
+// host images kept alive in `keep` until the caller has drained `stream`.
+struct ScratchArena {
+  unsigned char *base = nullptr;
+  size_t cap = 0, top = 0;
+  size_t last_off = 0;
+  void *last_ptr = nullptr;
+  hipStream_t stream = nullptr;
+  std::vector<std::vector<unsigned char>> keep;
+  std::map<std::pair<uint32_t, int>, void *> tw;  // (length, precision) -> table
+  // 256-byte-aligned slice of `bytes`, or nullptr when the region is exhausted
+  void *carve(size_t bytes);
+  bool owns(const void *p) const {
+    return p >= (const void *)base && p < (const void *)(base + cap);
+  }
+};
+void set_scratch_arena(ScratchArena *a);  // nullptr uninstalls
+ScratchArena *scratch_arena();
 // Host -> device copy of a caller's (pageable) buffer.  Large buffers go through
 // per-device pinned staging with several host threads and return once the copy
 // has landed (after the work already queued on st); small ones are one copy

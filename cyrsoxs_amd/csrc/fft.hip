@@ -382,18 +382,8 @@ bool make_stage_plan(uint32_t n, StagePlan &P, uint32_t &lcm) {
 
 size_t device_max_lds() { return max_lds_bytes(); }
 
-template <typename T> const cx<T> *twiddles(uint32_t n, cyr_status *st) {
-  int dev = 0;
-  hipGetDevice(&dev);
-  std::lock_guard<std::mutex> lk(g_mu);
-  auto key = std::make_tuple(dev, n, prec_of<T>());
-  auto it = g_tw.find(key);
-  if (it != g_tw.end()) {
-    *st = CYR_OK;
-    return reinterpret_cast<const cx<T> *>(it->second.ptr);
-  }
-  const size_t len = tw_len<T>(n);
-  std::vector<cx<T>> h(len);
+// Host image of the length-n table (tw_len<T>(n) entries).
+template <typename T> void twiddle_host(uint32_t n, cx<T> *h) {
   for (uint32_t m = 0; m < n; m++) {
     // exact-angle FP64 twiddle, rounded once to the compute precision; FP32
     // tables also keep the rounding residual (block 2)
@@ -406,6 +396,49 @@ template <typename T> const cx<T> *twiddles(uint32_t n, cyr_status *st) {
       h[n + m].y = (T)(sn - (double)h[m].y);
     }
   }
+}
+
+// Under an installed scratch arena the table is built into the arena for this
+// call (queued on the arena's stream), never into the process-wide cache.
+template <typename T> const cx<T> *arena_twiddles(ScratchArena &a, uint32_t n, cyr_status *st) {
+  const auto key = std::make_pair(n, prec_of<T>());
+  auto it = a.tw.find(key);
+  if (it != a.tw.end()) {
+    *st = CYR_OK;
+    return reinterpret_cast<const cx<T> *>(it->second);
+  }
+  const size_t bytes = sizeof(cx<T>) * tw_len<T>(n);
+  void *d = a.carve(bytes);
+  if (!d) {
+    *st = fail(CYR_E_NOMEM, "allocation-free call: twiddle tables exceed the scratch array");
+    return nullptr;
+  }
+  a.keep.emplace_back(bytes);
+  twiddle_host<T>(n, reinterpret_cast<cx<T> *>(a.keep.back().data()));
+  if (hipMemcpyAsync(d, a.keep.back().data(), bytes, hipMemcpyHostToDevice, a.stream) != hipSuccess) {
+    (void)hipGetLastError();
+    *st = fail(CYR_E_DEVICE, "twiddle upload failed");
+    return nullptr;
+  }
+  a.tw[key] = d;
+  *st = CYR_OK;
+  return reinterpret_cast<const cx<T> *>(d);
+}
+
+template <typename T> const cx<T> *twiddles(uint32_t n, cyr_status *st) {
+  if (ScratchArena *a = scratch_arena()) return arena_twiddles<T>(*a, n, st);
+  int dev = 0;
+  hipGetDevice(&dev);
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto key = std::make_tuple(dev, n, prec_of<T>());
+  auto it = g_tw.find(key);
+  if (it != g_tw.end()) {
+    *st = CYR_OK;
+    return reinterpret_cast<const cx<T> *>(it->second.ptr);
+  }
+  const size_t len = tw_len<T>(n);
+  std::vector<cx<T>> h(len);
+  twiddle_host<T>(n, h.data());
   DeviceBuffer b;
   cyr_status s = ensure_buffer(b, sizeof(cx<T>) * len);
   if (s != CYR_OK) {

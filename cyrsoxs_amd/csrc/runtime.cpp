@@ -34,8 +34,45 @@ cyr_status hip_fail(hipError_t e, const char *what, const char *file, int line) 
   return CYR_E_DEVICE;
 }
 
+static thread_local ScratchArena *t_arena = nullptr;
+
+void set_scratch_arena(ScratchArena *a) { t_arena = a; }
+ScratchArena *scratch_arena() { return t_arena; }
+
+void *ScratchArena::carve(size_t bytes) {
+  const size_t off = (top + 255) & ~(size_t)255;
+  if (off > cap || bytes > cap - off) return nullptr;
+  top = off + bytes;
+  last_off = off;
+  last_ptr = base + off;
+  return last_ptr;
+}
+
+// ensure_buffer under an installed arena: slices only, never an allocation.
+static cyr_status ensure_in_arena(ScratchArena &a, DeviceBuffer &b, size_t bytes) {
+  if (b.ptr && b.bytes >= bytes) return CYR_OK;
+  if (b.ptr && b.ptr == a.last_ptr && a.last_off + bytes <= a.cap) {  // topmost slice: grow in place
+    a.top = a.last_off + bytes;
+    b.bytes = bytes;
+    return CYR_OK;
+  }
+  void *p = a.carve(bytes);
+  if (!p) {
+    char buf[200];
+    std::snprintf(buf, sizeof(buf),
+                  "allocation-free call: its work buffers need more than the %zu-byte scratch array "
+                  "(%zu bytes in use, %zu more requested)",
+                  a.cap, a.top, bytes);
+    return fail(CYR_E_NOMEM, buf);
+  }
+  b.ptr = p;
+  b.bytes = bytes;
+  return CYR_OK;
+}
+
 cyr_status ensure_buffer(DeviceBuffer &b, size_t bytes) {
   if (bytes == 0) bytes = 256;
+  if (t_arena) return ensure_in_arena(*t_arena, b, bytes);
   if (b.ptr && b.bytes >= bytes) return CYR_OK;
   if (b.ptr) {
     hipFree(b.ptr);
@@ -56,7 +93,7 @@ cyr_status ensure_buffer(DeviceBuffer &b, size_t bytes) {
 }
 
 void free_buffer(DeviceBuffer &b) {
-  if (b.ptr) hipFree(b.ptr);
+  if (b.ptr && !(t_arena && t_arena->owns(b.ptr))) hipFree(b.ptr);
   b.ptr = nullptr;
   b.bytes = 0;
 }

@@ -103,6 +103,23 @@ cyr_status cyr_simulate(const cyr_sim_desc *desc, const void *voxels, const doub
 cyr_status cyr_simulate_device(const cyr_sim_desc *desc, const void *d_voxels, const double *mats,
                                void *d_out, void *stream);
 
+/* Allocation-free form of cyr_simulate_device, with the same arguments and
+ * results.  No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  It calls no hipMalloc/hipFree and no
+ * allocating library routine: the constant tables, twiddle tables and work
+ * buffers live in one static device array of CYR_NOALLOC_SCRATCH_BYTES per GPU,
+ * held from library load until the process ends.  Simulations that need more
+ * (e.g. 64^3 fits, 512^2 x 128 does not) are rejected with CYR_E_INVALID before
+ * any device work.  Every call on one GPU shares that GPU's array, which keeps
+ * the last call's contents (each call initialises what it reads); calls on one
+ * GPU from several threads are serialised by a lock, and each returns after
+ * `stream` has drained.  Not for use under graph capture of `stream` (the call
+ * synchronises it). */
+#define CYR_NOALLOC_SCRATCH_BYTES (128u * 1024u * 1024u)
+cyr_status cyr_simulate_device_noalloc(const cyr_sim_desc *desc, const void *d_voxels, const double *mats,
+                                       void *d_out, void *stream);
+
 /* Replaces int computePolarization(...) (include/cudaMain.h:84-87,
  * src/cudaMain.cu:1669-1738): one polarization for (energy_id, eangle_rad),
  * k vector 0, copied to three host complex arrays [V] of the compute precision. */

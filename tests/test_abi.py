@@ -172,7 +172,7 @@ def _dev_bufs(sim, vdtype=torch.float32, odtype=torch.float32):
 MATS = np.tile(np.array([1 - 1e-3, 1e-3, 1 - 1e-3, 1e-3]), (2, 2, 1))
 
 
-@pytest.mark.parametrize("fn", ["simulate_device"])
+@pytest.mark.parametrize("fn", ["simulate_device", "simulate_device_noalloc"])
 def test_device_api_rejects_bad_buffers(fn):
     sim = _small()
     call = getattr(capi, fn)
@@ -234,3 +234,12 @@ def test_batch_rejects_mismatched_lists():
         capi.simulate_batch([sim], [vox], [MATS], [out])
     assert "not on a GPU" in str(e.value)
 
+
+def test_noalloc_rejects_beyond_scratch_without_a_device():
+    """The size gate runs on the host before any device call."""
+    sim = capi.Sim((512, 512, 128), 4, 2.0, [285.0], (0.0, 30.0, 150.0), pipeline=capi.PIPE_MBASIS)
+    buf = np.zeros(64)
+    p = ctypes.c_void_p(buf.ctypes.data)
+    rc = capi.lib().cyr_simulate_device_noalloc(sim.ref, p, p, p, None)
+    assert rc == 1
+    assert b"scratch" in capi.lib().cyr_last_error()
