@@ -7,6 +7,8 @@
 
 #include <cstdint>
 #include <map>
+#include <mutex>
+#include <shared_mutex>
 #include <string>
 #include <type_traits>
 #include <utility>
@@ -214,6 +216,16 @@ struct ScratchArena {
 };
 void set_scratch_arena(ScratchArena *a);  // nullptr uninstalls
 ScratchArena *scratch_arena();
+
+// Process-wide device caches (twiddle tables, per-device work buffers) are shared
+// by every call that may use them (workspace_use(), held for the call) and freed
+// only by cyr_release_workspace under workspace_exclusive().  Objects that keep
+// pointers into them beyond one call (batch plans, serving engines) are counted
+// as live handles; release is refused while any exists.
+std::shared_lock<std::shared_mutex> workspace_use();
+std::unique_lock<std::shared_mutex> workspace_exclusive();
+void live_handles_add(int delta);
+int live_handles();
 // Host -> device copy of a caller's (pageable) buffer.  Large buffers go through
 // per-device pinned staging with several host threads and return once the copy
 // has landed (after the work already queued on st); small ones are one copy

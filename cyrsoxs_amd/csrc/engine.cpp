@@ -205,6 +205,15 @@ struct BatchPlan {
   hipStream_t cap = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  // also what a failed cyr_batch_create leaves behind
+  ~BatchPlan() {
+    if (cap) (void)hipStreamSynchronize(cap);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    lanes.destroy();
+    free_buffer(sched.tables);
+    if (cap) (void)hipStreamDestroy(cap);
+  }
 };
 
 }  // namespace cyr
@@ -224,6 +233,7 @@ cyr_status cyr_simulate_batch(uint32_t count, const cyr_sim_desc *descs, const v
   for (uint32_t i = 0; i < count; i++)
     if (!d_voxels[i] || !mats[i] || !d_outs[i]) return fail(CYR_E_INVALID, "null batch member buffer");
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  auto ws = workspace_use();
   BatchState *bs;
   CYR_TRY(batch_state(default_lanes(count), &bs));
   std::lock_guard<std::mutex> lk(bs->mu);
@@ -251,6 +261,7 @@ cyr_status cyr_batch_create(uint32_t count, const cyr_sim_desc *descs, const voi
   if (!out) return fail(CYR_E_INVALID, "null output handle");
   *out = nullptr;
   if (count == 0 || !descs || !d_voxels || !mats || !d_outs) return fail(CYR_E_INVALID, "empty or null batch");
+  auto ws = workspace_use();
   std::unique_ptr<cyr_batch_s> b(new cyr_batch_s());
   BatchPlan &p = b->p;
   CYR_HIP(hipGetDevice(&p.device));
@@ -293,6 +304,7 @@ cyr_status cyr_batch_create(uint32_t count, const cyr_sim_desc *descs, const voi
   if (ce != hipSuccess) return hip_fail(ce, "hipStreamEndCapture", __FILE__, __LINE__);
   p.graph = g;
   CYR_HIP(hipGraphInstantiate(&p.exec, p.graph, nullptr, nullptr, 0));
+  live_handles_add(1);  // the graph holds twiddle and work-buffer pointers
   *out = b.release();
   return CYR_OK;
 }
@@ -305,14 +317,8 @@ cyr_status cyr_batch_launch(cyr_batch b, void *stream) {
 
 cyr_status cyr_batch_destroy(cyr_batch b) {
   if (!b) return CYR_OK;
-  BatchPlan &p = b->p;
-  if (p.cap) (void)hipStreamSynchronize(p.cap);
-  if (p.exec) (void)hipGraphExecDestroy(p.exec);
-  if (p.graph) (void)hipGraphDestroy(p.graph);
-  p.lanes.destroy();
-  free_buffer(p.sched.tables);
-  if (p.cap) (void)hipStreamDestroy(p.cap);
-  delete b;
+  delete b;  // ~BatchPlan drains and frees
+  live_handles_add(-1);
   return CYR_OK;
 }
 
@@ -344,6 +350,15 @@ struct Slot {
   WorkBuffers wb;
   DeviceBuffer vox, out;
   std::thread th;
+  // device bytes the slot's buffers hold between requests, and the bytes
+  // counted against the budget for it (held, plus a running request's footprint)
+  uint64_t held = 0, reserved = 0;
+  uint64_t bytes_now() const { return vox.bytes + out.bytes + wb.bytes(); }
+  void release_buffers() {
+    wb.release();
+    free_buffer(vox);
+    free_buffer(out);
+  }
 };
 
 }  // namespace
@@ -384,26 +399,60 @@ cyr_status Engine::run_request(Slot *s, Request &r) {
   return CYR_OK;
 }
 
+// Admission (FIFO): `resident` sums every slot's reservation, so buffers an idle
+// slot keeps from an earlier request count against the budget too.  A slot takes
+// the queue head when its footprint fits next to the other slots' reservations
+// and what this slot holds (buffers only grow, so the run may hold both); when it
+// fits only without this slot's buffers, the slot frees them first; when it does
+// not fit at all, an idle slot that holds buffers frees them, since they may be
+// what keeps the head out.  With every slot idle and empty the head always fits
+// (submit rejects footprints above the budget), so the queue keeps draining.
 void Engine::slot_loop(Slot *s) {
   (void)hipSetDevice(device);
   for (;;) {
     std::shared_ptr<Request> r;
+    bool shrink = false, admit = false;
     {
       std::unique_lock<std::mutex> lk(mu);
-      // FIFO admission: the head of the queue runs once its footprint fits
-      cv_work.wait(lk, [&] {
-        if (stop) return true;
-        if (queue.empty()) return false;
-        const auto &h = reqs[queue.front()];
-        return resident + h->footprint <= budget;
-      });
-      if (stop) return;
-      r = reqs[queue.front()];
-      queue.pop_front();
-      r->state = kRunning;
-      r->t_start = Clock::now();
-      resident += r->footprint;
-      in_flight++;
+      for (;;) {
+        if (stop) return;
+        if (!queue.empty()) {
+          const auto &h = reqs[queue.front()];
+          const uint64_t other = resident - s->reserved;
+          if (other + s->held + h->footprint <= budget) {
+            admit = true;
+          } else if (other + h->footprint <= budget) {
+            admit = shrink = true;
+          } else if (s->held > 0) {
+            shrink = true;
+          }
+        }
+        if (admit || shrink) break;
+        cv_work.wait(lk);
+      }
+      if (admit) {
+        r = reqs[queue.front()];
+        queue.pop_front();
+        r->state = kRunning;
+        r->t_start = Clock::now();
+        in_flight++;
+        const uint64_t res = (shrink ? 0 : s->held) + r->footprint;
+        resident = resident - s->reserved + res;
+        s->reserved = res;
+      }
+    }
+    if (shrink) {  // this slot's stream is idle: nothing queued on it uses the buffers
+      s->release_buffers();
+      std::lock_guard<std::mutex> lk(mu);
+      s->held = 0;
+      if (!admit) {
+        resident -= s->reserved;
+        s->reserved = 0;
+      }
+    }
+    if (!admit) {
+      cv_work.notify_all();
+      continue;
     }
     const cyr_status st = run_request(s, *r);
     {
@@ -411,7 +460,9 @@ void Engine::slot_loop(Slot *s) {
       r->status = st;
       r->state = st == CYR_OK ? kDone : kFailed;
       r->t_done = Clock::now();
-      resident -= r->footprint;
+      s->held = s->bytes_now();
+      resident = resident - s->reserved + s->held;
+      s->reserved = s->held;
       in_flight--;
       completed++;
     }
@@ -458,6 +509,7 @@ cyr_status cyr_engine_create(int device, uint64_t budget_bytes, uint32_t slots, 
     Slot *sp = s.get();
     sp->th = std::thread([&e, sp] { e.slot_loop(sp); });
   }
+  live_handles_add(1);  // slot threads use the shared device caches while it lives
   *out = h.release();
   return CYR_OK;
 }
@@ -607,12 +659,11 @@ cyr_status cyr_engine_destroy(cyr_engine eng) {
   (void)hipSetDevice(e.device);
   for (auto &s : e.slots) {
     (void)hipStreamSynchronize(s->st);
-    s->wb.release();
-    free_buffer(s->vox);
-    free_buffer(s->out);
+    s->release_buffers();
     (void)hipStreamDestroy(s->st);
   }
   delete eng;
+  live_handles_add(-1);
   return CYR_OK;
 }
 

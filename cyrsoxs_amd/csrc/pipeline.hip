@@ -491,6 +491,7 @@ cyr_status mbasis_band(const cyr_sim_desc *d, const SimGeometry &g, uint32_t k, 
   CYR_HIP(hipMemcpyAsync(&h, bmax, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   CYR_HIP(hipStreamSynchronize(s));
   std::lock_guard<std::mutex> lk(g_band_mu);
+  if (g_band_widths.size() >= 4096) g_band_widths.clear();  // bounded: entries are cheap to recompute
   g_band_widths[key] = h;
   *J = h;
   return CYR_OK;
@@ -1014,6 +1015,7 @@ uint32_t cyr_energy_lanes(void) { return energy_lanes(); }
 cyr_status cyr_autotune(const cyr_sim_desc *desc, cyr_yz_tuning *chosen) {
   SimGeometry g;
   CYR_TRY(build_geometry(desc, g));
+  auto ws = workspace_use();
   DeviceState *ds;
   CYR_TRY(device_state(&ds));
   std::lock_guard<std::mutex> lk(ds->mu);
@@ -1047,6 +1049,7 @@ cyr_status cyr_simulate_device(const cyr_sim_desc *desc, const void *d_voxels, c
   SimGeometry g;
   CYR_TRY(build_geometry(desc, g));
   if (!d_voxels || !mats || !d_out) return fail(CYR_E_INVALID, "null buffer");
+  auto ws = workspace_use();
   DeviceState *ds;
   CYR_TRY(device_state(&ds));
   std::lock_guard<std::mutex> lk(ds->mu);
@@ -1058,6 +1061,7 @@ cyr_status cyr_simulate(const cyr_sim_desc *desc, const void *voxels, const doub
   SimGeometry g;
   CYR_TRY(build_geometry(desc, g));
   if (!voxels || !mats || !out) return fail(CYR_E_INVALID, "null buffer");
+  auto ws = workspace_use();
   DeviceState *ds;
   CYR_TRY(device_state(&ds));
   std::lock_guard<std::mutex> lk(ds->mu);
@@ -1085,6 +1089,7 @@ cyr_status cyr_polarization_device(const cyr_sim_desc *desc, const void *d_voxel
   if (desc->case_type != 0)  // src/cudaMain.cu:1677-1680
     return fail(CYR_E_UNSUPPORTED, "Only implemented for Case Type = 0");
   if (!d_voxels || !mats_e || !d_px || !d_py || !d_pz) return fail(CYR_E_INVALID, "null buffer");
+  auto ws = workspace_use();
   DeviceState *ds;
   CYR_TRY(device_state(&ds));
   std::lock_guard<std::mutex> lk(ds->mu);
@@ -1172,6 +1177,7 @@ cyr_status cyr_fft3d_device(uint32_t nx, uint32_t ny, uint32_t nz, int32_t preci
   if (!d_volumes || nx < 1 || ny < 1 || nz < 1) return fail(CYR_E_INVALID, "bad fft arguments");
   for (uint32_t i = 0; i < count; i++)
     if (!d_volumes[i]) return fail(CYR_E_INVALID, "null volume");
+  auto ws = workspace_use();
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   if (precision == CYR_FP64) CYR_TRY(fft3d<double>(d_volumes, count, nx, ny, nz, st));
   else if (precision == CYR_FP32) CYR_TRY(fft3d<float>(d_volumes, count, nx, ny, nz, st));
@@ -1185,6 +1191,7 @@ cyr_status cyr_fft_axis_device(uint32_t nx, uint32_t ny, uint32_t nz, int32_t pr
   if (!d_volumes || nx < 1 || ny < 1 || nz < 1 || axis < 0 || axis > 2) return fail(CYR_E_INVALID, "bad fft arguments");
   for (uint32_t i = 0; i < count; i++)
     if (!d_volumes[i]) return fail(CYR_E_INVALID, "null volume");
+  auto ws = workspace_use();
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   if (precision == CYR_FP64) return fft_axis<double>(d_volumes, count, nx, ny, nz, axis, st);
   if (precision == CYR_FP32) return fft_axis<float>(d_volumes, count, nx, ny, nz, axis, st);
@@ -1262,6 +1269,13 @@ cyr_status cyr_warp_affine_device(uint32_t nx, uint32_t ny, int32_t precision, c
 }
 
 cyr_status cyr_release_workspace(void) {
+  auto ws = workspace_exclusive();  // waits for calls that use the caches
+  if (live_handles() > 0)
+    return fail(CYR_E_INVALID, "a batch plan or serving engine is alive: destroy it before releasing the workspace");
+  {
+    std::lock_guard<std::mutex> lb(g_band_mu);
+    g_band_widths.clear();
+  }
   std::lock_guard<std::mutex> lk(g_states_mu);
   int cur = 0;
   hipGetDevice(&cur);

@@ -7,6 +7,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <shared_mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -33,6 +34,14 @@ cyr_status hip_fail(hipError_t e, const char *what, const char *file, int line) 
   if (e == hipErrorNoDevice || e == hipErrorInvalidDevice) return CYR_E_NODEVICE;
   return CYR_E_DEVICE;
 }
+
+static std::shared_mutex g_workspace_mu;
+static std::atomic<int> g_live_handles{0};
+
+std::shared_lock<std::shared_mutex> workspace_use() { return std::shared_lock<std::shared_mutex>(g_workspace_mu); }
+std::unique_lock<std::shared_mutex> workspace_exclusive() { return std::unique_lock<std::shared_mutex>(g_workspace_mu); }
+void live_handles_add(int delta) { g_live_handles.fetch_add(delta); }
+int live_handles() { return g_live_handles.load(); }
 
 static thread_local ScratchArena *t_arena = nullptr;
 

@@ -194,7 +194,8 @@ typedef struct cyr_engine_s *cyr_engine;
 
 typedef struct {
   uint64_t submitted, completed, rejected, cancelled;
-  uint64_t resident_bytes;   /* HBM reserved by the requests now running */
+  uint64_t resident_bytes;   /* HBM counted against the budget: what the slots' buffers hold
+                                between requests plus the footprints of requests now running */
   uint64_t budget_bytes;
   uint32_t in_flight;
   uint32_t queued;
@@ -273,7 +274,12 @@ uint32_t cyr_num_angles(double start, double increment, double end);
 int32_t cyr_resolved_pipeline(const cyr_sim_desc *desc);
 /* Device bytes a simulation holds while it runs (morphology + work volumes). */
 uint64_t cyr_footprint_bytes(const cyr_sim_desc *desc);
-/* Frees the per-device cached work buffers of the plain/E-basis executors. */
+/* Frees the library's per-device caches (work buffers, twiddle tables, batch
+ * streams, upload staging).  It waits for calls that are using them and returns
+ * CYR_E_INVALID while a batch plan or serving engine exists (their graphs and
+ * threads keep pointers into the caches): destroy those first.  Work queued
+ * through the library must have finished (cyr_fft_axis_device returns before
+ * its kernels do). */
 cyr_status cyr_release_workspace(void);
 /* Library version string. */
 const char *cyr_version(void);

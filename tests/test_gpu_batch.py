@@ -114,30 +114,84 @@ def test_engine_results_alone_and_in_mix():
             _same(eng.release(rid), ref)
         st = eng.stats()
         assert st["submitted"] == len(members) and st["completed"] == len(members)
-        assert st["in_flight"] == 0 and st["queued"] == 0 and st["resident_bytes"] == 0
+        assert st["in_flight"] == 0 and st["queued"] == 0
+        # idle slots keep their buffers, and they stay counted against the budget
+        assert 0 < st["resident_bytes"] <= st["budget_bytes"]
+
+
+def _blocker():
+    """A plain-pipeline request that keeps one slot busy for a while (128^3, 4
+    energies x 36 angles: 144 polarization + 3 FFT passes)."""
+    sim, vox, mats = _member(7, dict(dims=(128, 128, 128), M=2, morph=0, pipeline=capi.PIPE_PLAIN,
+                                     kw=dict(interpolation=1)), nE=4)
+    sim = capi.Sim(sim.dims, 2, 4.0, list(sim.energies), (0.0, 5.0, 175.0), pipeline=capi.PIPE_PLAIN, interpolation=1)
+    return sim, vox, mats
 
 
 def test_engine_admission_cancel_and_timeout():
     s, v, m = _member(4, MEMBERS[4], nE=12)
-    with capi.Engine(slots=1, budget_bytes=s.footprint() + 1024) as eng:
-        big, vb, mb = _member(4, dict(MEMBERS[4], dims=(96, 96, 48)))
+    bs, bv, bm = _blocker()
+    with capi.Engine(slots=1) as eng:
+        with capi.Engine(slots=1, budget_bytes=s.footprint() + 1024) as small:
+            big, vb, mb = _member(4, dict(MEMBERS[4], dims=(96, 96, 48)))
+            with pytest.raises(capi.CyrError) as e:
+                small.submit(big, vb, mb)
+            assert "ADMISSION" in str(e.value)
+            assert small.stats()["rejected"] == 1
+        # the only slot runs the blocker; the next request waits in the queue
+        blocker = eng.submit(bs, bv, bm)
+        queued = eng.submit(s, v, m)
+        last = eng.submit(s, v, m)
         with pytest.raises(capi.CyrError) as e:
-            eng.submit(big, vb, mb)
-        assert "ADMISSION" in str(e.value)
-        ids = [eng.submit(s, v, m) for _ in range(3)]
-        # with one slot and room for one request, the last is still queued
-        try:
-            eng.cancel(ids[-1])
-            cancelled = True
-        except capi.CyrError:
-            cancelled = False
-        for rid in ids[:-1]:
-            assert eng.wait(rid, timeout=60.0) == 0
-        st_last = eng.wait(ids[-1], timeout=60.0)
-        assert st_last == (5 if cancelled else 0)  # CYR_E_CANCELLED
-        ref = _alone(s, v, m)
-        for rid in ids[:-1]:
-            _same(eng.result(rid), ref)
+            eng.wait(queued, timeout=0.0)
+        assert e.value.status == 8  # CYR_E_TIMEOUT: still queued
+        eng.cancel(last)
+        assert eng.wait(last, timeout=60.0) == 5  # CYR_E_CANCELLED
+        assert eng.wait(blocker, timeout=120.0) == 0
+        assert eng.wait(queued, timeout=60.0) == 0
+        _same(eng.result(queued), _alone(s, v, m))
         stats = eng.stats()
-        assert stats["rejected"] == 1
-        assert stats["resident_bytes"] == 0
+        assert stats["cancelled"] == 1 and stats["completed"] == 2 and stats["queued"] == 0
+
+
+def test_engine_counts_retained_buffers():
+    """A large request then small ones, with a budget that holds the large one's
+    footprint only: the slot that ran it frees its buffers before a small one runs
+    (otherwise the device would hold both), every request succeeds, and the
+    reported reservation always covers what the idle slots hold."""
+    big, vb, mb = _member(4, dict(MEMBERS[4], dims=(96, 96, 48)), nE=3)
+    small = [_member(i, MEMBERS[i], nE=2) for i in (0, 1, 2)]
+    held_big = vb.nbytes + 3 * 96 * 96 * 4  # its morphology and patterns at least
+    with capi.Engine(slots=2, budget_bytes=big.footprint() + 1024) as eng:
+        rid = eng.submit(big, vb, mb)
+        assert eng.wait(rid, timeout=60.0) == 0
+        st = eng.stats()
+        assert held_big <= st["resident_bytes"] <= st["budget_bytes"]
+        _same(eng.release(rid), _alone(big, vb, mb))
+        ids = [eng.submit(sm, v, m) for sm, v, m in small]
+        for (sm, v, m), i in zip(small, ids):
+            assert eng.wait(i, timeout=60.0) == 0
+            _same(eng.release(i), _alone(sm, v, m))
+        st = eng.stats()
+        assert 0 < st["resident_bytes"] <= st["budget_bytes"]
+        assert st["completed"] == 4 and st["in_flight"] == 0
+
+
+def test_release_workspace_refused_while_plan_or_engine_lives():
+    members = [_member(0, MEMBERS[0])]
+    d_vox = [torch.from_numpy(v).cuda() for _, v, _ in members]
+    d_out = [torch.empty((len(s.energies), 1, s.dims[1], s.dims[0]), dtype=torch.float32, device="cuda")
+             for s, _, _ in members]
+    plan = capi.BatchPlan([s for s, _, _ in members], d_vox, [m for _, _, m in members], d_out)
+    try:
+        assert capi.lib().cyr_release_workspace() == 1  # CYR_E_INVALID: the graph holds cache pointers
+        assert b"batch plan" in capi.lib().cyr_last_error()
+    finally:
+        plan.close()
+    with capi.Engine(slots=1):
+        assert capi.lib().cyr_release_workspace() == 1
+    assert capi.lib().cyr_release_workspace() == 0
+    # the library rebuilds its caches on the next call
+    s, v, m = members[0]
+    _same(_alone(s, v, m), O.simulate(O.Sim(s.dims, 2, 4.0, list(s.energies), (0.0, 15.0, 165.0),
+                                            **MEMBERS[0]["kw"]), v.astype(np.float64), m))

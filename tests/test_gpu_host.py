@@ -24,7 +24,7 @@ cy = pytest.importorskip("CyRSoXS")
 
 
 @pytest.mark.parametrize("precision,tol", [("FP32", 1e-4), ("FP64", 1e-10)])
-@pytest.mark.parametrize("pipeline", ["EBasis", "Plain"])
+@pytest.mark.parametrize("pipeline", ["Auto", "MBasis", "EBasis", "Plain"])
 def test_notebook_flow_matches_oracle(precision, tol, pipeline):
     n = 32
     vox = synthetic.sphere_morphology(n, 10.0)  # [2][V][4] (S, theta, psi, Vfrac)

@@ -143,7 +143,8 @@ def test_host_upload_matches_device_api():
     half = capi.simulate(capi.Sim(dims, M, 5.0, energies, (0.0, 15.0, 165.0), storage=capi.STORE_FP16, **kw),
                          vox.numpy(), mats)
     err, same = pattern_rel_l2(half, dev)
-    assert same and err < 2e-2, err
+    print(f"host upload FP16 storage vs FP32: rel L2 {err:.3e}")
+    assert same and err < 2e-3, err
 
 
 @pytest.mark.parametrize("prec", [capi.FP64, capi.FP32])
@@ -187,9 +188,10 @@ def test_repeatable_bitwise():
         np.testing.assert_array_equal(a, b)
 
 
-@pytest.mark.parametrize("storage,tol", [(capi.STORE_FP16, 2e-2), (capi.STORE_BF16, 1e-1)])
+@pytest.mark.parametrize("storage,tol", [(capi.STORE_FP16, 1e-4), (capi.STORE_BF16, 4e-4)])
 def test_half_precision_morphology_storage(storage, tol):
-    """FP16/BF16 morphology storage (FP32 compute): masks identical, error reported."""
+    """FP16/BF16 morphology storage (FP32 compute): masks identical; bars about 10x the
+    measured errors (FP16 9.3e-6, BF16 3.3e-5 relative L2 on MI355X)."""
     dims, M = (32, 32, 16), 3
     vox = random_euler(dims, M, 31)
     energies = [284.0, 286.0]
@@ -305,11 +307,11 @@ def test_autotune_records_a_choice():
 
 
 @pytest.mark.parametrize("prec,storage,tol", [(capi.FP64, capi.STORE_NATIVE, 1e-10), (capi.FP32, capi.STORE_NATIVE, 1e-4),
-                                              (capi.FP32, capi.STORE_FP16, 2e-2), (capi.FP32, capi.STORE_BF16, 1e-1)])
+                                              (capi.FP32, capi.STORE_FP16, 3e-4), (capi.FP32, capi.STORE_BF16, 3e-3)])
 def test_thinfilm_precision(prec, storage, tol):
     """The thin-film morphology (3 phases, in-plane director) at 256^2 x 1 with the
     full 0:2:358 rotation against the FP64 oracle; FP16/BF16 morphology storage
-    carries its own (reported) error."""
+    carries its own error (measured 2.1e-5 and 2.5e-4 relative L2; bars about 10x)."""
     n = 256
     vox = synthetic.thinfilm_morphology(n, seed=3).numpy().astype(np.float64)
     energies = [284.0, 286.5]
