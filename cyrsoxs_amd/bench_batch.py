@@ -30,15 +30,23 @@ import sys
 import time
 
 
-def main(argv=None):
+def parse(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--count", type=int, default=64)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--reps", type=int, default=2)
     ap.add_argument("--plain-count", type=int, default=16)
     ap.add_argument("--pipeline", choices=("mbasis", "ebasis"), default="mbasis")
-    a = ap.parse_args(argv)
+    return ap.parse_args(argv)
 
+
+def main(argv=None):
+    print(json.dumps(run(parse(argv))), flush=True)
+    return 0
+
+
+def run(a):
+    """The batched workload's modes; returns the result object (the JSON line)."""
     import numpy as np
     import torch
 
@@ -153,8 +161,9 @@ def main(argv=None):
         "speedup_batch_vs_serial": res[f"{pn}_batch"]["value"] / res[f"{pn}_serial"]["value"],
         "speedup_graph_vs_serial": res[f"{pn}_graph"]["value"] / res[f"{pn}_serial"]["value"],
     }
-    print(json.dumps(out), flush=True)
-    return 0
+    del vox, outs
+    torch.cuda.empty_cache()
+    return out
 
 
 if __name__ == "__main__":

@@ -39,7 +39,7 @@ def _pct(xs, p):
     return s[k]
 
 
-def main(argv=None):
+def parse(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--rates", default="2,5,10,20,40")
     ap.add_argument("--window", type=float, default=15.0)
@@ -50,8 +50,16 @@ def main(argv=None):
     ap.add_argument("--fifo-rate", type=float, default=5.0)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--pipeline", choices=("auto", "mbasis", "ebasis"), default="auto")
-    a = ap.parse_args(argv)
+    return ap.parse_args(argv)
 
+
+def main(argv=None):
+    print(json.dumps(run(parse(argv))), flush=True)
+    return 0
+
+
+def run(a):
+    """Serves the Poisson streams; returns the result object (the JSON line)."""
     import numpy as np
     import torch
 
@@ -168,8 +176,8 @@ def main(argv=None):
                        "Hanning, FP32",
            "pipeline": a.pipeline, "continuous_batching": results, f"{a.pipeline}_fifo_one_slot": fifo,
            "plain_fifo_one_slot": plain}
-    print(json.dumps(out), flush=True)
-    return 0
+    del pool
+    return out
 
 
 if __name__ == "__main__":
