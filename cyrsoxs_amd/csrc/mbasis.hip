@@ -369,7 +369,7 @@ __global__ void __launch_bounds__(256) mbasis_quad_kernel(MbAnglesArgs<T> a) {
   cx<T> F[NG][kEbFields];
   double md[NG][6];
   T w1;
-  T *o = a.quad + ((uint64_t)e * P2 + t) * 4;
+  T *o = a.quad + ((uint64_t)e * P2 + quad_index(X, Y, a.g.nx, a.g.ny)) * 4;
   if (!mb_fields<T, NG>(a, X, Y, F, md, w1)) {
     o[0] = o[1] = o[2] = mb_qnan<T>();
     o[3] = 0;

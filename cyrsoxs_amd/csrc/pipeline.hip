@@ -100,7 +100,8 @@ inline uint32_t orbit_tiles(const SimGeometry &g) {
 
 // Per-simulation constant tables, uploaded once.
 struct Tables {
-  size_t mats = 0, han = 0, amat = 0, rotinv = 0, detinv = 0, kmag = 0, quad = 0, tiles = 0, otiles = 0, total = 0;
+  size_t mats = 0, han = 0, amat = 0, rotinv = 0, detinv = 0, kmag = 0, quad = 0, quad_t = 0, tiles = 0, otiles = 0,
+         total = 0;
   std::vector<unsigned char> host;
 };
 
@@ -165,6 +166,8 @@ template <typename T> void build_layout(const SimGeometry &g, Tables &tb) {
   off = align256(off + sizeof(double) * g.nE);
   tb.quad = off;
   off = align256(off + sizeof(double) * 3 * g.nK * g.nA);
+  tb.quad_t = off;
+  off = align256(off + sizeof(T) * 3 * g.nK * g.nA);
   tb.tiles = off;
   off = align256(off + sizeof(uint32_t) * rotation_tile_count(g.nx, g.ny));
   tb.otiles = off;
@@ -197,11 +200,13 @@ void build_tables(const cyr_sim_desc *d, const SimGeometry &g, const double *mat
   std::memcpy(tb.host.data() + tb.kmag, g.kmag.data(), sizeof(double) * g.nE);
   // laboratory frame: I(theta) = c^2 q0 + s^2 q1 + 2 c s q2 with E = c e1 + s e2
   double *qc = reinterpret_cast<double *>(tb.host.data() + tb.quad);
+  T *qt = reinterpret_cast<T *>(tb.host.data() + tb.quad_t);
   for (size_t i = 0; i < (size_t)g.nK * g.nA; i++) {
     const double c = g.ebasis[2 * i], sn = g.ebasis[2 * i + 1];
     qc[3 * i + 0] = c * c;
     qc[3 * i + 1] = sn * sn;
     qc[3 * i + 2] = 2.0 * c * sn;
+    for (int j = 0; j < 3; j++) qt[3 * i + j] = (T)qc[3 * i + j];
   }
   rotation_tile_order(g.nx, g.ny, reinterpret_cast<uint32_t *>(tb.host.data() + tb.tiles));
   if (orbit_tiles(g)) rotation_orbit_tile_order(g.nx, reinterpret_cast<uint32_t *>(tb.host.data() + tb.otiles));
@@ -219,6 +224,7 @@ template <typename T> struct Setup {
   const MatConst<T> *mats;
   const T *hx, *hy, *hz, *amat;
   const double *rotinv, *kmag, *quad;
+  const T *quad_t;
   const uint32_t *tiles, *otiles;  // otiles: null without quarter-turn orbits
   const cx<T> *twy, *twz;
   double vox_bytes, cxb;
@@ -240,6 +246,7 @@ cyr_status setup_views(const cyr_sim_desc *d, const SimGeometry &g, const Tables
   su.rotinv = reinterpret_cast<const double *>(tbase + tb.rotinv);
   su.kmag = reinterpret_cast<const double *>(tbase + tb.kmag);
   su.quad = reinterpret_cast<const double *>(tbase + tb.quad);
+  su.quad_t = reinterpret_cast<const T *>(tbase + tb.quad_t);
   su.tiles = reinterpret_cast<const uint32_t *>(tbase + tb.tiles);
   su.otiles = orbit_tiles(g) ? reinterpret_cast<const uint32_t *>(tbase + tb.otiles) : nullptr;
   cyr_status s;
@@ -515,6 +522,22 @@ uint32_t mbasis_band_cap(const SimGeometry &g) {
   return cap > floor_cap ? cap : floor_cap;
 }
 
+// Whether the quadratic-form coefficients (c^2, s^2, 2cs) of every angle a of the
+// first quarter turn equal, rounded to the compute precision, those of a + 2
+// quarter turns (and a + A4 those of a + 3 quarter turns): E turned by 180 degrees
+// gives the same intensity, so the orbit kernel forms one value for both pairs.
+template <typename T> bool quad_pairs_share(const SimGeometry &g, uint32_t k) {
+  const uint32_t A4 = g.quarter;
+  if (A4 == 0 || g.nA != 4 * A4) return false;
+  for (uint32_t a = 0; a < 2 * A4; a++) {
+    const double *x = &g.ebasis[2 * ((size_t)k * g.nA + a)], *y = &g.ebasis[2 * ((size_t)k * g.nA + a + 2 * A4)];
+    const double cx = x[0], sx = x[1], cy = y[0], sy = y[1];
+    if ((T)(cx * cx) != (T)(cy * cy) || (T)(sx * sx) != (T)(sy * sy) || (T)(2.0 * cx * sx) != (T)(2.0 * cy * sy))
+      return false;
+  }
+  return true;
+}
+
 // MBASIS: per (k, energy group) the seven real basis fields of every material
 // (packed two per complex volume), their 3D FFTs, and their values in the band
 // of Ewald bins the group samples (plus the mirrored bins); then per energy the
@@ -662,8 +685,9 @@ cyr_status run_mbasis(const cyr_sim_desc *d, const SimGeometry &g, const void *d
           // column, shared by its energies); outputs: n quadratic-form planes
           kt_end("mbasis_quad", s, 2.0 * nvol * (double)J * P * cxb + n * 4.0 * P * sizeof(T));
           kt_begin(s);
-          CYR_TRY(launch_rotate_quad<T>(proj, inv, su.quad + (size_t)3 * k * g.nA, g.nA, g.nx, g.ny, d->rot_mask, avg,
-                                        n, su.tiles, g.quarter, su.otiles, s));
+          CYR_TRY(launch_rotate_quad<T>(proj, inv, su.quad + (size_t)3 * k * g.nA, su.quad_t + (size_t)3 * k * g.nA,
+                                        g.nA, g.nx, g.ny, d->rot_mask, avg, n, su.tiles, g.quarter, su.otiles,
+                                        quad_pairs_share<T>(g, k), s));
           kt_end("rotate_quad", s, n * (4.0 * P * sizeof(T) + 8.0 * P));
           CYR_TRY(launch_detector_warp_n<T>(avg, d_out + ((uint64_t)b0 * g.nK + k) * P, (uint64_t)g.nK * P, n, g.nx,
                                             g.ny, &g.det_inv[(size_t)k * 6], d->rot_mask, s));

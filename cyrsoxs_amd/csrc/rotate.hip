@@ -140,7 +140,8 @@ template <typename T, int NE>
 __device__ inline void quad_gather(const Quad4<T> *const (&q4)[NE], const QPoint &p, uint32_t nx, uint32_t ny,
                                    Quad4<T> (&tp)[NE][4]) {
   const uint32_t x1 = p.x0 + 1 < nx ? p.x0 + 1 : p.x0, y1 = p.y0 + 1 < ny ? p.y0 + 1 : p.y0;
-  const uint32_t off[4] = {p.y0 * nx + p.x0, p.y0 * nx + x1, y1 * nx + p.x0, y1 * nx + x1};  // P < 2^32
+  const uint32_t off[4] = {quad_index(p.x0, p.y0, nx, ny), quad_index(x1, p.y0, nx, ny), quad_index(p.x0, y1, nx, ny),
+                           quad_index(x1, y1, nx, ny)};  // P < 2^32
 #pragma unroll
   for (int e = 0; e < NE; e++)
 #pragma unroll
@@ -261,25 +262,57 @@ __global__ void __launch_bounds__(256) rotate_quad_kernel(const T *__restrict__ 
 // [1, N-1]^2 minus the centre): it back-maps and gathers each (s^i x, a < A4) once
 // and serves the Q (pixel, angle) pairs that sample there.
 //
-// The pairs' own FP64 back-maps differ from the shared point by rounding of the
-// angle tables (~N 1e-15 pixels: at most 4e-12 over every pair of a 4096^2 plane
-// with 720 angles).  When the shared point lies farther than the margin
-// 1e-9 + N 1e-12 from every cell edge and from the image border, all Q pairs have its tap cell,
-// validity and nonzero weights, so they reuse its bilinear sums (only the
-// weights' last bits can differ from a pair's own); otherwise (a point on or
-// near a pixel edge: angles at exact multiples of 90 degrees) each pair
-// back-maps its own point and gathers its own taps when its cell differs, as
-// quad_pixel does.  Validity and NaN taps are therefore decided exactly as per
-// pair; only the order of the FP64 sums over angles differs.  Row 0, column 0
-// and the centre (the tail blocks after the orbit tiles) take every angle
-// directly.
+// The pairs' own FP64 back-maps (contraction off) differ from the shared point,
+// evaluated with fused multiply-adds, by the rounding of the angle tables and
+// of the evaluation (~N 1e-15 pixels: at most 4e-12 over every pair of a 4096^2
+// plane with 720 angles).  When the shared point lies farther than the margin
+// 1e-9 + N 1e-12 from every cell edge and from the image border, all Q pairs have
+// its tap cell, validity and nonzero weights, so they reuse its bilinear sums
+// (only the weights' last bits can differ from a pair's own).  Such points take
+// the lean path: no zero-weight tests, taps through 32-bit buffer offsets (the
+// plane is below 2 GiB), the intensity in the compute precision, and one value
+// for the pairs q and q + 2 when their coefficients are equal (`share`).  The
+// rest (a point on or near a pixel edge: angles at exact multiples of 90 degrees)
+// is recorded per angle and replayed after the main loop: there each pair
+// back-maps its own point and gathers its own taps, as quad_pixel does.
+// Validity and NaN taps are therefore decided exactly as per pair; only the order
+// of the FP64 sums over angles differs.  Without the rotation mask a point
+// outside the image marks its pairs' pixels NaN (the value the sum would reach).
+// Row 0, column 0 and the centre (the tail blocks after the orbit tiles) take
+// every angle directly.
 
-template <typename T, int NE, int Q>
+template <typename T> struct Tap3;
+template <> struct Tap3<float> { typedef float v __attribute__((ext_vector_type(3))); };
+template <> struct Tap3<double> { typedef double v __attribute__((ext_vector_type(3))); };
+
+// (q0, q1, q2) of the record at byte offset `off` of a plane; offsets past the
+// buffer's size read zeros (only taps of points that take the exact path do).
+template <typename T> __device__ inline typename Tap3<T>::v tap_load(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  if constexpr (sizeof(T) == 4) {
+    return __builtin_amdgcn_raw_buffer_load_b96(r, off, 0, 0);
+  } else {
+    typedef int i4 __attribute__((ext_vector_type(4)));
+    typedef int i2 __attribute__((ext_vector_type(2)));
+    const i4 a = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+    const i2 b = __builtin_amdgcn_raw_buffer_load_b64(r, off + 16, 0, 0);
+    typename Tap3<double>::v v;
+    v.x = __builtin_bit_cast(double, ((unsigned long long)(unsigned)a.y << 32) | (unsigned)a.x);
+    v.y = __builtin_bit_cast(double, ((unsigned long long)(unsigned)a.w << 32) | (unsigned)a.z);
+    v.z = __builtin_bit_cast(double, ((unsigned long long)(unsigned)b.y << 32) | (unsigned)b.x);
+    return v;
+  }
+}
+
+// Buffer resource word 3 of a raw (untyped) buffer on gfx9-family devices.
+constexpr int kRawBufferFlags = 0x00020000;
+
+template <typename T, int NE, int Q, bool RM>
 __global__ void __launch_bounds__(256) rotate_orbit_kernel(const T *__restrict__ quad, const double *__restrict__ inv_tab,
-                                                           const double *__restrict__ coef, uint32_t nA, uint32_t A4,
-                                                           uint32_t n, int rot_mask, double *__restrict__ avg,
-                                                           uint32_t ne, const uint32_t *__restrict__ otiles,
-                                                           uint32_t n_otiles) {
+                                                           const double *__restrict__ coef,
+                                                           const T *__restrict__ coef_t, uint32_t nA, uint32_t A4,
+                                                           uint32_t n, double *__restrict__ avg, uint32_t ne,
+                                                           const uint32_t *__restrict__ otiles, uint32_t n_otiles,
+                                                           int share) {
   const uint64_t P = (uint64_t)n * n;
   const uint32_t h = n / 2;
   const uint32_t qb = xcd_tile(blockIdx.x, gridDim.x);
@@ -295,8 +328,8 @@ __global__ void __launch_bounds__(256) rotate_orbit_kernel(const T *__restrict__
     const uint32_t Y = j < n ? 0 : (j < 2 * n - 1 ? j - n + 1 : h);
     double acc[NE] = {};
     uint32_t cnt[NE] = {};
-    quad_pixel<T, NE>(q4, inv_tab, coef, nA, n, n, X, Y, rot_mask, acc, cnt);
-    quad_store<NE>(avg, P, (uint64_t)Y * n + X, e0, ne, nA, rot_mask, acc, cnt);
+    quad_pixel<T, NE>(q4, inv_tab, coef, nA, n, n, X, Y, RM, acc, cnt);
+    quad_store<NE>(avg, P, (uint64_t)Y * n + X, e0, ne, nA, RM, acc, cnt);
     return;
   }
   const uint32_t tt = otiles[qb];
@@ -309,45 +342,129 @@ __global__ void __launch_bounds__(256) rotate_orbit_kernel(const T *__restrict__
   const uint32_t PX[4] = {X0, n - Y0, n - X0, Y0};
   const uint32_t PY[4] = {Y0, X0, n - Y0, n - X0};
   const double hi = (double)(n - 1);
+  const double m = 1e-9 + 1e-12 * (double)n;
+  constexpr uint32_t rb = (uint32_t)sizeof(Quad4<T>);
+  __amdgpu_buffer_rsrc_t rs[NE];
+#pragma unroll
+  for (int e = 0; e < NE; e++)
+    rs[e] = __builtin_amdgcn_make_buffer_rsrc((void *)q4[e], (short)0, (int)(P * rb), kRawBufferFlags);
   double acc[4][NE] = {};
   uint32_t cnt[4][NE] = {};
+  bool dead[4] = {false, false, false, false};
   static_for<0, 4>([&](auto ic) {
     constexpr int i = decltype(ic)::value;
+    uint64_t defer = 0;  // angles (bit a) whose point takes the exact path
     for (uint32_t a = 0; a < A4; a++) {
-      const QPoint b = quad_point(inv_tab + 6 * (uint64_t)a, PX[i], PY[i], n, n);
-      const double m = 1e-9 + 1e-12 * (double)n;
-      const bool safe = b.inside ? (b.fx >= m && b.fx <= 1 - m && b.fy >= m && b.fy <= 1 - m)
-                                 : (b.sx < -m || b.sx > hi + m || b.sy < -m || b.sy > hi + m);
-      Quad4<T> tp[NE][4];
-      quad_gather<T, NE>(q4, b, n, n, tp);
-      if (safe) {
-        T sm[NE][3];
-        quad_sums<T, NE>(b, tp, sm);
-        static_for<0, Q>([&](auto qc) {
-          constexpr int q = decltype(qc)::value;
-          constexpr int j = (i - q + 4) & 3;  // s^q(pixel j) = pixel i
-          quad_combine<T, NE>(b.inside, sm, coef + 3 * (uint64_t)(a + q * A4), rot_mask, acc[j], cnt[j]);
-        });
-      } else {
-        quad_add<T, NE>(b, tp, coef + 3 * (uint64_t)a, rot_mask, acc[i], cnt[i]);
-        static_for<1, Q>([&](auto qc) {
-          constexpr int q = decltype(qc)::value;
-          constexpr int j = (i - q + 4) & 3;
-          const uint32_t aa = a + q * A4;
-          const QPoint d = quad_point(inv_tab + 6 * (uint64_t)aa, PX[j], PY[j], n, n);
-          if (d.inside && !(b.inside && d.x0 == b.x0 && d.y0 == b.y0)) {
-            Quad4<T> td[NE][4];
-            quad_gather<T, NE>(q4, d, n, n, td);
-            quad_add<T, NE>(d, td, coef + 3 * (uint64_t)aa, rot_mask, acc[j], cnt[j]);
+      const double *inv = inv_tab + 6 * (uint64_t)a;
+      const double sx = __builtin_fma(inv[0], (double)PX[i], __builtin_fma(inv[1], (double)PY[i], inv[2]));
+      const double sy = __builtin_fma(inv[3], (double)PX[i], __builtin_fma(inv[4], (double)PY[i], inv[5]));
+      const bool inside = sx >= 0 && sx <= hi && sy >= 0 && sy <= hi;
+      const double fx0 = floor(sx), fy0 = floor(sy);
+      const double fx = sx - fx0, fy = sy - fy0;
+      const bool safe = inside ? (fx >= m && fx <= 1 - m && fy >= m && fy <= 1 - m)
+                               : (sx < -m || sx > hi + m || sy < -m || sy > hi + m);
+      const uint32_t x0 = inside ? (uint32_t)fx0 : 0u, y0 = inside ? (uint32_t)fy0 : 0u;
+      const uint32_t o00 = quad_index(x0, y0, n, n) * rb, o10 = quad_index(x0 + 1, y0, n, n) * rb;
+      const uint32_t o01 = quad_index(x0, y0 + 1, n, n) * rb, o11 = quad_index(x0 + 1, y0 + 1, n, n) * rb;
+      typename Tap3<T>::v t[NE][4];
+#pragma unroll
+      for (int e = 0; e < NE; e++) {
+        t[e][0] = tap_load<T>(rs[e], o00);
+        t[e][1] = tap_load<T>(rs[e], o10);
+        t[e][2] = tap_load<T>(rs[e], o01);
+        t[e][3] = tap_load<T>(rs[e], o11);
+      }
+      if (!safe) {
+        defer |= 1ull << a;
+        continue;
+      }
+      // every pair shares this point's cell, validity and nonzero weights
+      const T fxT = (T)fx, fyT = (T)fy, gx = (T)1 - fxT, gy = (T)1 - fyT;
+      const T w00 = gx * gy, w10 = fxT * gy, w01 = gx * fyT, w11 = fxT * fyT;
+      T sm[NE][3];
+#pragma unroll
+      for (int e = 0; e < NE; e++) {
+        sm[e][0] = w00 * t[e][0].x + w10 * t[e][1].x + w01 * t[e][2].x + w11 * t[e][3].x;
+        sm[e][1] = w00 * t[e][0].y + w10 * t[e][1].y + w01 * t[e][2].y + w11 * t[e][3].y;
+        sm[e][2] = w00 * t[e][0].z + w10 * t[e][1].z + w01 * t[e][2].z + w11 * t[e][3].z;
+      }
+      auto add = [&](auto qc, const T(&vq)[NE]) {
+        constexpr int q = decltype(qc)::value;
+        constexpr int j = (i - q + 4) & 3;  // s^q(pixel j) = pixel i
+#pragma unroll
+        for (int e = 0; e < NE; e++) {
+          if constexpr (RM) {
+            const double vv = (double)vq[e];
+            const bool ok = inside && !isnan(vv);
+            acc[j][e] += ok ? vv : 0.0;
+            cnt[j][e] += ok ? 1u : 0u;
           } else {
-            quad_add<T, NE>(d, tp, coef + 3 * (uint64_t)aa, rot_mask, acc[j], cnt[j]);
+            acc[j][e] += (double)vq[e];
           }
+        }
+        if constexpr (!RM) dead[j] = dead[j] || !inside;
+      };
+      auto value = [&](int q, T(&vq)[NE]) {
+        const T *c = coef_t + 3 * (uint64_t)(a + q * A4);
+#pragma unroll
+        for (int e = 0; e < NE; e++) vq[e] = c[0] * sm[e][0] + c[1] * sm[e][1] + c[2] * sm[e][2];
+      };
+      bool shared = false;
+      if constexpr (Q == 4) {
+        if (share) {  // pairs q and q + 2 have equal coefficients
+          T v0[NE], v1[NE];
+          value(0, v0);
+          value(1, v1);
+          add(std::integral_constant<int, 0>{}, v0);
+          add(std::integral_constant<int, 1>{}, v1);
+          add(std::integral_constant<int, 2>{}, v0);
+          add(std::integral_constant<int, 3>{}, v1);
+          shared = true;
+        }
+      }
+      if (!shared) {
+        static_for<0, Q>([&](auto qc) {
+          T vq[NE];
+          value(decltype(qc)::value, vq);
+          add(qc, vq);
         });
+      }
+    }
+    // points near a cell edge or the image border: every pair back-maps its own
+    // point exactly and gathers its own taps (one pair at a time, so the rare path
+    // holds few registers)
+    while (defer) {
+      const uint32_t a = (uint32_t)__builtin_ctzll(defer);
+      defer &= defer - 1;
+#pragma unroll 1
+      for (uint32_t q = 0; q < (uint32_t)Q; q++) {
+        const uint32_t j = (i - q + 4) & 3;
+        const uint32_t aa = a + q * A4;
+        const QPoint d = quad_point(inv_tab + 6 * (uint64_t)aa, PX[j], PY[j], n, n);
+        Quad4<T> td[NE][4];
+        quad_gather<T, NE>(q4, d, n, n, td);
+        T sm[NE][3];
+        quad_sums<T, NE>(d, td, sm);
+        const double *c = coef + 3 * (uint64_t)aa;
+        switch (j) {  // constant indices keep the accumulators in registers
+          case 0: quad_combine<T, NE>(d.inside, sm, c, RM, acc[0], cnt[0]); break;
+          case 1: quad_combine<T, NE>(d.inside, sm, c, RM, acc[1], cnt[1]); break;
+          case 2: quad_combine<T, NE>(d.inside, sm, c, RM, acc[2], cnt[2]); break;
+          default: quad_combine<T, NE>(d.inside, sm, c, RM, acc[3], cnt[3]); break;
+        }
       }
     }
   });
 #pragma unroll
-  for (int i = 0; i < 4; i++) quad_store<NE>(avg, P, (uint64_t)PY[i] * n + PX[i], e0, ne, nA, rot_mask, acc[i], cnt[i]);
+  for (int i = 0; i < 4; i++) {
+    if constexpr (!RM) {
+      if (dead[i]) {
+#pragma unroll
+        for (int e = 0; e < NE; e++) acc[i][e] = __longlong_as_double(0x7ff8000000000000ll);
+      }
+    }
+    quad_store<NE>(avg, P, (uint64_t)PY[i] * n + PX[i], e0, ne, nA, RM, acc[i], cnt[i]);
+  }
 }
 
 // grid y = pattern: avg + y P -> out + y out_stride
@@ -450,35 +567,37 @@ cyr_status launch_rotate_all(const T *proj, const double *inv_tab, uint32_t nA, 
   return CYR_OK;
 }
 
-template <typename T, int NE>
-hipError_t launch_orbit(uint32_t Q, dim3 grid, hipStream_t st, const T *quad, const double *inv_tab,
-                        const double *coef, uint32_t nA, uint32_t A4, uint32_t n, int rot_mask, double *avg,
-                        uint32_t ne, const uint32_t *otiles, uint32_t n_otiles) {
+template <typename T, int NE, bool RM>
+hipError_t launch_orbit_rm(uint32_t Q, dim3 grid, hipStream_t st, const T *quad, const double *inv_tab,
+                           const double *coef, const T *coef_t, uint32_t nA, uint32_t A4, uint32_t n, double *avg,
+                           uint32_t ne, const uint32_t *otiles, uint32_t n_otiles, int share) {
   switch (Q) {
     case 2:
-      hipLaunchKernelGGL((rotate_orbit_kernel<T, NE, 2>), grid, dim3(256), 0, st, quad, inv_tab, coef, nA, A4, n,
-                         rot_mask, avg, ne, otiles, n_otiles);
+      hipLaunchKernelGGL((rotate_orbit_kernel<T, NE, 2, RM>), grid, dim3(256), 0, st, quad, inv_tab, coef, coef_t, nA,
+                         A4, n, avg, ne, otiles, n_otiles, share);
       break;
     case 3:
-      hipLaunchKernelGGL((rotate_orbit_kernel<T, NE, 3>), grid, dim3(256), 0, st, quad, inv_tab, coef, nA, A4, n,
-                         rot_mask, avg, ne, otiles, n_otiles);
+      hipLaunchKernelGGL((rotate_orbit_kernel<T, NE, 3, RM>), grid, dim3(256), 0, st, quad, inv_tab, coef, coef_t, nA,
+                         A4, n, avg, ne, otiles, n_otiles, share);
       break;
     default:
-      hipLaunchKernelGGL((rotate_orbit_kernel<T, NE, 4>), grid, dim3(256), 0, st, quad, inv_tab, coef, nA, A4, n,
-                         rot_mask, avg, ne, otiles, n_otiles);
+      hipLaunchKernelGGL((rotate_orbit_kernel<T, NE, 4, RM>), grid, dim3(256), 0, st, quad, inv_tab, coef, coef_t, nA,
+                         A4, n, avg, ne, otiles, n_otiles, share);
       break;
   }
   return hipGetLastError();
 }
 
 template <typename T>
-cyr_status launch_rotate_quad(const T *quad, const double *inv_tab, const double *coef, uint32_t nA, uint32_t nx,
-                              uint32_t ny, int rot_mask, double *avg, uint32_t ne, const uint32_t *tiles,
-                              uint32_t quarter, const uint32_t *otiles, hipStream_t st) {
+cyr_status launch_rotate_quad(const T *quad, const double *inv_tab, const double *coef, const T *coef_t, uint32_t nA,
+                              uint32_t nx, uint32_t ny, int rot_mask, double *avg, uint32_t ne, const uint32_t *tiles,
+                              uint32_t quarter, const uint32_t *otiles, bool share, hipStream_t st) {
   if (ne == 0) return CYR_OK;
   if ((uint64_t)nx * ny > 0xffffffffull) return fail(CYR_E_UNSUPPORTED, "rotation: more than 2^32 pixels");
   if (!tiles) return fail(CYR_E_INVALID, "rotation: tile order missing");
-  const uint32_t Q = otiles ? rotation_quarter_turns(nx, ny, nA, quarter) : 0;
+  // the orbit kernel addresses a plane through 32-bit buffer offsets
+  const bool small_plane = (uint64_t)nx * ny * sizeof(Quad4<T>) < (1ull << 31);
+  const uint32_t Q = otiles && small_plane ? rotation_quarter_turns(nx, ny, nA, quarter) : 0;
   if (Q) {
     // quarter-turn orbits: each gathered tap cell serves Q (pixel, angle) pairs
     const uint32_t n_otiles = rotation_orbit_tile_count(nx);
@@ -487,7 +606,10 @@ cyr_status launch_rotate_quad(const T *quad, const double *inv_tab, const double
     const uint64_t blocks = (uint64_t)n_otiles + ceil_div(2ull * nx, 256);
     const dim3 grid(xcd_grid(blocks), 1, ceil_div(ne, 2));
     const hipError_t e =
-        launch_orbit<T, 2>(Q, grid, st, quad, inv_tab, coef, nA, quarter, nx, rot_mask, avg, ne, otiles, n_otiles);
+        rot_mask ? launch_orbit_rm<T, 2, true>(Q, grid, st, quad, inv_tab, coef, coef_t, nA, quarter, nx, avg, ne,
+                                                otiles, n_otiles, share ? 1 : 0)
+                 : launch_orbit_rm<T, 2, false>(Q, grid, st, quad, inv_tab, coef, coef_t, nA, quarter, nx, avg, ne,
+                                                 otiles, n_otiles, share ? 1 : 0);
     if (e != hipSuccess) return hip_fail(e, "rotate_orbit_kernel", __FILE__, __LINE__);
     return CYR_OK;
   }
@@ -539,9 +661,9 @@ cyr_status launch_warp(const T *src, T *dst, uint32_t nx, uint32_t ny, const dou
                                                   const double *, int, hipStream_t);                               \
   template cyr_status launch_rotate_all<T>(const T *, const double *, uint32_t, uint32_t, uint32_t, int, double *, \
                                            const uint32_t *, hipStream_t);                                         \
-  template cyr_status launch_rotate_quad<T>(const T *, const double *, const double *, uint32_t, uint32_t,        \
-                                            uint32_t, int, double *, uint32_t, const uint32_t *, uint32_t,         \
-                                            const uint32_t *, hipStream_t);                                        \
+  template cyr_status launch_rotate_quad<T>(const T *, const double *, const double *, const T *, uint32_t,      \
+                                            uint32_t, uint32_t, int, double *, uint32_t, const uint32_t *,        \
+                                            uint32_t, const uint32_t *, bool, hipStream_t);                        \
   template cyr_status launch_detector_warp_n<T>(const double *, T *, uint64_t, uint32_t, uint32_t, uint32_t,       \
                                                 const double *, int, hipStream_t);                                 \
   template cyr_status launch_detector_warp<T>(const double *, T *, uint32_t, uint32_t, const double *, int,        \
