@@ -142,8 +142,8 @@ template <typename T> cyr_status launch_ebasis_angles(const AnglesArgs<T> &a, in
 // Packed basis volumes per k: 7 per material pair, 4 for an odd last material.
 inline uint32_t mbasis_volumes(uint32_t M) { return 7 * (M / 2) + 4 * (M % 2); }
 // Per column (FFT-index order): Zlo | (width << 16) of the shifted z indices the
-// energies [0, ne) sample (kmag[e] = |k|); bmax[0] = widest band, bmax[1..] scratch
-// of mbasis_bmax_words - 1 entries.
+// energies [0, ne) sample (kmag[e] = |k|); bmax[0] = widest band, bmax[1..2] = sum
+// of the columns' widths (uint64), then per-block scratch (mbasis_bmax_words in all).
 cyr_status launch_mbasis_bands(const EwaldGeom &g, const double *kmag, uint32_t ne, int32_t *band, uint32_t *bmax,
                                hipStream_t st);
 uint64_t mbasis_bmax_words(uint32_t nx, uint32_t ny);

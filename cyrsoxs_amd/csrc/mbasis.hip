@@ -37,10 +37,11 @@ template <> __device__ inline double mb_qnan<double>() { return __longlong_as_do
 // ---- band of shifted z indices per column over the energy group ---------------
 // band[col] = Zlo | (width << 16) over every energy's Ewald z index (and its
 // linear-interpolation partner z + 1); 0 = no energy samples the column.  Each
-// block also writes the widest band it saw (no atomics: a second kernel reduces).
+// block also writes the widest band and the sum of the widths it saw (no atomics:
+// a second kernel reduces): bmax[3 + 2b] = widest, bmax[4 + 2b] = sum of block b.
 __global__ void __launch_bounds__(256) mbasis_bands_kernel(EwaldGeom g, const double *__restrict__ kmag, uint32_t ne,
                                                            int32_t *__restrict__ band, uint32_t *__restrict__ bmax) {
-  __shared__ uint32_t red[256];
+  __shared__ uint32_t red[2][256];
   const uint64_t P = (uint64_t)g.nx * g.ny;
   const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t w = 0;
@@ -62,27 +63,48 @@ __global__ void __launch_bounds__(256) mbasis_bands_kernel(EwaldGeom g, const do
     const uint64_t col = (uint64_t)unshift(Y, g.ny) * g.nx + unshift(X, g.nx);
     band[col] = w ? (int32_t)(lo | (w << 16)) : 0;
   }
-  red[threadIdx.x] = w;
+  red[0][threadIdx.x] = w;
+  red[1][threadIdx.x] = w;
   __syncthreads();
   for (uint32_t s = blockDim.x / 2; s > 0; s >>= 1) {
-    if (threadIdx.x < s) red[threadIdx.x] = red[threadIdx.x + s] > red[threadIdx.x] ? red[threadIdx.x + s] : red[threadIdx.x];
+    if (threadIdx.x < s) {
+      red[0][threadIdx.x] = red[0][threadIdx.x + s] > red[0][threadIdx.x] ? red[0][threadIdx.x + s] : red[0][threadIdx.x];
+      red[1][threadIdx.x] += red[1][threadIdx.x + s];
+    }
     __syncthreads();
   }
-  if (threadIdx.x == 0) bmax[1 + blockIdx.x] = red[0];
+  if (threadIdx.x == 0) {
+    bmax[3 + 2 * blockIdx.x] = red[0][0];
+    bmax[4 + 2 * blockIdx.x] = red[1][0];
+  }
 }
 
-// bmax[0] = max over the n per-block entries bmax[1..n]
+// bmax[0] = widest band, bmax[1..2] = sum of all widths (uint64, little endian)
+// over the n per-block entries.
 __global__ void __launch_bounds__(256) mbasis_bmax_kernel(uint32_t *__restrict__ bmax, uint32_t n) {
   __shared__ uint32_t red[256];
+  __shared__ unsigned long long sum[256];
   uint32_t m = 0;
-  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) m = bmax[1 + i] > m ? bmax[1 + i] : m;
+  unsigned long long s = 0;
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+    m = bmax[3 + 2 * i] > m ? bmax[3 + 2 * i] : m;
+    s += bmax[4 + 2 * i];
+  }
   red[threadIdx.x] = m;
+  sum[threadIdx.x] = s;
   __syncthreads();
-  for (uint32_t s = blockDim.x / 2; s > 0; s >>= 1) {
-    if (threadIdx.x < s) red[threadIdx.x] = red[threadIdx.x + s] > red[threadIdx.x] ? red[threadIdx.x + s] : red[threadIdx.x];
+  for (uint32_t k = blockDim.x / 2; k > 0; k >>= 1) {
+    if (threadIdx.x < k) {
+      red[threadIdx.x] = red[threadIdx.x + k] > red[threadIdx.x] ? red[threadIdx.x + k] : red[threadIdx.x];
+      sum[threadIdx.x] += sum[threadIdx.x + k];
+    }
     __syncthreads();
   }
-  if (threadIdx.x == 0) bmax[0] = red[0];
+  if (threadIdx.x == 0) {
+    bmax[0] = red[0];
+    bmax[1] = (uint32_t)(sum[0] & 0xffffffffull);
+    bmax[2] = (uint32_t)(sum[0] >> 32);
+  }
 }
 
 // ---- band values of the transformed packed volumes ----------------------------
@@ -430,7 +452,7 @@ cyr_status launch_mbasis_bands(const EwaldGeom &g, const double *kmag, uint32_t 
   return CYR_OK;
 }
 
-uint64_t mbasis_bmax_words(uint32_t nx, uint32_t ny) { return 1 + ceil_div((uint64_t)nx * ny, 256); }
+uint64_t mbasis_bmax_words(uint32_t nx, uint32_t ny) { return 3 + 2 * ceil_div((uint64_t)nx * ny, 256); }
 
 template <typename T> cyr_status launch_mbasis_extract(const BandArgs<T> &a, hipStream_t st) {
   if (a.count == 0) return CYR_OK;

@@ -466,16 +466,19 @@ cyr_status run_ebasis(const cyr_sim_desc *d, const SimGeometry &g, const void *d
   return CYR_OK;
 }
 
-// Widest Ewald band over energies [e0, e0 + ne) for k, by (grid, k, |k| list):
-// known after one eager run, so graph capture can size its launches.
+// Widest Ewald band and the sum of the columns' band widths over energies
+// [e0, e0 + ne) for k, by (grid, k, |k| list): known after one eager run, so graph
+// capture can size its launches.
 std::mutex g_band_mu;
-std::map<std::vector<double>, uint32_t> g_band_widths;
+std::map<std::vector<double>, std::pair<uint32_t, uint64_t>> g_band_widths;
 
-// Band table of energies [e0, e0 + ne) for k vector k into `band`, and its widest
-// band in *J (read back once per distinct geometry, which waits for the stream;
-// a stream under graph capture needs it cached by an earlier run).
+// Band table of energies [e0, e0 + ne) for k vector k into `band`, its widest band
+// in *J and the sum of the widths in *wsum (read back once per distinct geometry,
+// which waits for the stream; a stream under graph capture needs them cached by
+// an earlier run).
 cyr_status mbasis_band(const cyr_sim_desc *d, const SimGeometry &g, uint32_t k, uint32_t e0, uint32_t ne,
-                       const double *d_kmag, int32_t *band, uint32_t *bmax, hipStream_t s, uint32_t *J) {
+                       const double *d_kmag, int32_t *band, uint32_t *bmax, hipStream_t s, uint32_t *J,
+                       uint64_t *wsum) {
   const double kv[3] = {g.kvec[k].x, g.kvec[k].y, g.kvec[k].z};
   const EwaldGeom geo = make_ewald_geom(g.nx, g.ny, g.nz, d->phys_size, g.kmag[e0], kv, d->interpolation);
   CYR_TRY(launch_mbasis_bands(geo, d_kmag + e0, ne, band, bmax, s));
@@ -486,7 +489,8 @@ cyr_status mbasis_band(const cyr_sim_desc *d, const SimGeometry &g, uint32_t k, 
     std::lock_guard<std::mutex> lk(g_band_mu);
     auto it = g_band_widths.find(key);
     if (it != g_band_widths.end()) {
-      *J = it->second;
+      *J = it->second.first;
+      *wsum = it->second.second;
       return CYR_OK;
     }
   }
@@ -494,13 +498,15 @@ cyr_status mbasis_band(const cyr_sim_desc *d, const SimGeometry &g, uint32_t k, 
   CYR_HIP(hipStreamIsCapturing(s, &cap));
   if (cap != hipStreamCaptureStatusNone)
     return fail(CYR_E_INVALID, "material basis: Ewald band widths unknown under capture (run the simulation once first)");
-  uint32_t h = 0;
-  CYR_HIP(hipMemcpyAsync(&h, bmax, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  uint32_t h[3] = {0, 0, 0};
+  CYR_HIP(hipMemcpyAsync(h, bmax, sizeof(h), hipMemcpyDeviceToHost, s));
   CYR_HIP(hipStreamSynchronize(s));
+  const uint64_t sum = (uint64_t)h[1] | ((uint64_t)h[2] << 32);
   std::lock_guard<std::mutex> lk(g_band_mu);
   if (g_band_widths.size() >= 4096) g_band_widths.clear();  // bounded: entries are cheap to recompute
-  g_band_widths[key] = h;
-  *J = h;
+  g_band_widths[key] = {h[0], sum};
+  *J = h[0];
+  *wsum = sum;
   return CYR_OK;
 }
 
@@ -575,9 +581,10 @@ cyr_status run_mbasis(const cyr_sim_desc *d, const SimGeometry &g, const void *d
     const M3 Q = ebasis_frame(g, k);
     for (uint32_t e0 = 0; e0 < g.nE;) {
       uint32_t ne = g.nE - e0, J = 0;
+      uint64_t wsum = 0;  // sum over columns of the band widths
       for (;;) {
         kt_begin(s);
-        CYR_TRY(mbasis_band(d, g, k, e0, ne, su.kmag, band, bmax, s, &J));
+        CYR_TRY(mbasis_band(d, g, k, e0, ne, su.kmag, band, bmax, s, &J, &wsum));
         kt_end("mbasis_bands", s, (double)P * 4);
         if (J <= cap || ne == 1) break;
         ne = (ne + 1) / 2;
@@ -641,7 +648,9 @@ cyr_status run_mbasis(const cyr_sim_desc *d, const SimGeometry &g, const void *d
         if (three) {
           kt_begin(s);
           CYR_TRY(launch_mbasis_zband<T>(ba, s, &fused_z));
-          if (fused_z) kt_end("mbasis_zband", s, nv * (double)V * cxb);
+          // reads every volume once; writes each column's band values and those of
+          // its mirror column (own + mir)
+          if (fused_z) kt_end("mbasis_zband", s, nv * (double)V * cxb + 2.0 * nv * (double)wsum * cxb);
           if (!fused_z) {
             kt_begin(s);
             CYR_TRY(fft_axis<T>(vptr, nv, g.nx, g.ny, g.nz, 2, s));
@@ -681,9 +690,10 @@ cyr_status run_mbasis(const cyr_sim_desc *d, const SimGeometry &g, const void *d
           aa.quad = proj;
           kt_begin(s);
           CYR_TRY(launch_mbasis_angles<T>(aa, ng, s));
-          // unique inputs: the group's band values (own + mirror, at most J slots per
-          // column, shared by its energies); outputs: n quadratic-form planes
-          kt_end("mbasis_quad", s, 2.0 * nvol * (double)J * P * cxb + n * 4.0 * P * sizeof(T));
+          // unique inputs: the group's band values (own + mirror, each column's band
+          // width, shared by its energies) and the band table; outputs: n
+          // quadratic-form planes
+          kt_end("mbasis_quad", s, 2.0 * nvol * (double)wsum * cxb + 4.0 * P + n * 4.0 * P * sizeof(T));
           kt_begin(s);
           CYR_TRY(launch_rotate_quad<T>(proj, inv, su.quad + (size_t)3 * k * g.nA, su.quad_t + (size_t)3 * k * g.nA,
                                         g.nA, g.nx, g.ny, d->rot_mask, avg, n, su.tiles, g.quarter, su.otiles,
