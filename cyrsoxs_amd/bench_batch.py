@@ -113,8 +113,18 @@ def run(a):
         serial(PIPE, range(a.count))
         t = min(timed(lambda: serial(PIPE, range(a.count))) for _ in range(a.reps))
         w, same = compare()
+        # how much of a one-at-a-time pass the device spends inside kernels: the
+        # rest (launch gaps, host work between members, tails of small kernels) is
+        # all that running members concurrently or as one graph can reclaim
+        capi.kernel_timing(True)
+        t_busy = timed(lambda: serial(PIPE, range(a.count)))
+        capi.kernel_timing(False)
+        ksum = sum(k["total_us"] for k in capi.kernel_timing_report().values()) * 1e-6
         res[f"{pn}_serial"] = {"seconds": t, "value": total_units / t, "max_rel_l2_vs_ebasis": w,
-                               "nan_mask_identical": same}
+                               "nan_mask_identical": same, "kernel_seconds": ksum,
+                               "kernel_busy_fraction": ksum / t,
+                               "note": "kernel_seconds: sum of the launches' HIP-event durations in one serial pass "
+                                       f"({t_busy:.4f} s with the events)"}
 
     capi.simulate_batch(ps, vox, mats, outs)
     t = min(timed(lambda: capi.simulate_batch(ps, vox, mats, outs)) for _ in range(a.reps))
@@ -160,6 +170,7 @@ def run(a):
         "speedup_graph_vs_plain_serial": res[f"{pn}_graph"]["value"] / plain_v,
         "speedup_batch_vs_serial": res[f"{pn}_batch"]["value"] / res[f"{pn}_serial"]["value"],
         "speedup_graph_vs_serial": res[f"{pn}_graph"]["value"] / res[f"{pn}_serial"]["value"],
+        "batch_bound_vs_serial": 1.0 / res[f"{pn}_serial"]["kernel_busy_fraction"] if pn != "ebasis" else None,
     }
     del vox, outs
     torch.cuda.empty_cache()

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -79,6 +80,31 @@ static cyr_status ensure_in_arena(ScratchArena &a, DeviceBuffer &b, size_t bytes
   return CYR_OK;
 }
 
+// Test hook for the error path of a failed device allocation: with
+// CYR_TEST_FAIL_ALLOC=k in the environment, the device allocation after the next
+// k ones fails with CYR_E_NOMEM (and every later one, until the value changes)
+// without calling hipMalloc.
+static bool injected_alloc_failure() {
+  static std::mutex mu;
+  static std::string seen;
+  static long left = -1;
+  const char *v = std::getenv("CYR_TEST_FAIL_ALLOC");
+  std::lock_guard<std::mutex> lk(mu);
+  if (!v || !*v) {
+    seen.clear();
+    left = -1;
+    return false;
+  }
+  if (seen != v) {
+    seen = v;
+    left = std::atol(v);
+  }
+  if (left < 0) return false;
+  if (left == 0) return true;
+  left--;
+  return false;
+}
+
 cyr_status ensure_buffer(DeviceBuffer &b, size_t bytes) {
   if (bytes == 0) bytes = 256;
   if (t_arena) return ensure_in_arena(*t_arena, b, bytes);
@@ -88,6 +114,7 @@ cyr_status ensure_buffer(DeviceBuffer &b, size_t bytes) {
     b.ptr = nullptr;
     b.bytes = 0;
   }
+  if (injected_alloc_failure()) return fail(CYR_E_NOMEM, "injected device allocation failure (CYR_TEST_FAIL_ALLOC)");
   void *p = nullptr;
   hipError_t e = hipMalloc(&p, bytes);
   if (e != hipSuccess) {

@@ -195,3 +195,29 @@ def test_release_workspace_refused_while_plan_or_engine_lives():
     s, v, m = members[0]
     _same(_alone(s, v, m), O.simulate(O.Sim(s.dims, 2, 4.0, list(s.energies), (0.0, 15.0, 165.0),
                                             **MEMBERS[0]["kw"]), v.astype(np.float64), m))
+
+
+def test_injected_allocation_failure_is_reported_and_recovered(monkeypatch):
+    """A device allocation that fails (forced from the host through the allocator's
+    test hook, CYR_TEST_FAIL_ALLOC) makes the call return CYR_E_NOMEM; the library
+    and a serving engine keep working afterwards."""
+    s, v, m = _member(1, MEMBERS[1])
+    ref = O.simulate(O.Sim(s.dims, 3, 4.0, list(s.energies), (0.0, 15.0, 165.0), **MEMBERS[1]["kw"]),
+                     v.astype(np.float64), m)
+    assert capi.lib().cyr_release_workspace() == 0  # empty caches: the next call allocates
+    monkeypatch.setenv("CYR_TEST_FAIL_ALLOC", "2")  # the third allocation fails
+    with pytest.raises(capi.CyrError) as e:
+        capi.simulate(s, v, m)
+    assert e.value.status == 2 and "injected" in str(e.value)
+    monkeypatch.delenv("CYR_TEST_FAIL_ALLOC")
+    _same(capi.simulate(s, v, m), ref)
+    # in the engine: the failing request reports CYR_E_NOMEM, the next one is served
+    with capi.Engine(slots=1) as eng:
+        monkeypatch.setenv("CYR_TEST_FAIL_ALLOC", "0")
+        bad = eng.submit(s, v, m)
+        assert eng.wait(bad, timeout=60.0) == 2
+        monkeypatch.delenv("CYR_TEST_FAIL_ALLOC")
+        good = eng.submit(s, v, m)
+        assert eng.wait(good, timeout=60.0) == 0
+        _same(eng.result(good), ref)
+        assert eng.stats()["completed"] == 2
