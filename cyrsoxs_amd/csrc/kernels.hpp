@@ -187,11 +187,15 @@ template <typename T> struct MbAnglesArgs {
 // (laboratory frame), else every angle's projection of one energy.
 template <typename T> cyr_status launch_mbasis_angles(const MbAnglesArgs<T> &a, int ng, hipStream_t st);
 
-// Record of pixel (x, y) in a plane of quadratic forms (4 values per pixel): 4 x 4
-// pixel blocks, row-major inside the block and over blocks, when both sides are
-// multiples of 4; row-major otherwise.  A wave gathering the bilinear taps of a
-// rotated 8 x 8 pixel footprint then touches fewer cache lines than with
-// row-major records (measured: thin-film rotation 6.9 -> 5.5 ms per 10 energies).
+// Values per record of a plane of quadratic forms (q0, q1, q2 of one pixel, packed
+// without padding: a gather reads 12 of every 12 bytes it brings in, not 12 of 16).
+constexpr uint32_t kQuadRecord = 3;
+
+// Record of pixel (x, y) in a plane of quadratic forms: 4 x 4 pixel blocks,
+// row-major inside the block and over blocks, when both sides are multiples of 4;
+// row-major otherwise.  A wave gathering the bilinear taps of a rotated 8 x 8
+// pixel footprint then touches fewer cache lines than with row-major records
+// (measured: thin-film rotation 6.9 -> 5.5 ms per 10 energies).
 __host__ __device__ inline uint32_t quad_index(uint32_t x, uint32_t y, uint32_t nx, uint32_t ny) {
   if ((nx & 3u) == 0 && (ny & 3u) == 0) return ((y >> 2) * (nx >> 2) + (x >> 2)) * 16u + ((y & 3u) << 2) + (x & 3u);
   return y * nx + x;

@@ -374,7 +374,7 @@ __global__ void __launch_bounds__(256) mbasis_angles_kernel(MbAnglesArgs<T> a) {
 // ---- per energy, laboratory frame: the intensity's quadratic form ---------------
 // With E(theta) = c e1 + s e2 the polarization is p = c u + s v, u = Q (F0, F1, F2),
 // v = Q (F1, F3, F4), so I(theta) = c^2 |M u|^2 + s^2 |M v|^2 + 2 c s Re<M u, M v>.
-// quad[pixel] = (|Mu|^2, |Mv|^2, Re<Mu, Mv>, 0), interpolated between the bins.
+// quad[pixel] = (|Mu|^2, |Mv|^2, Re<Mu, Mv>), interpolated between the bins.
 template <typename T, int NG>
 __global__ void __launch_bounds__(256) mbasis_quad_kernel(MbAnglesArgs<T> a) {
   const uint64_t P2 = (uint64_t)a.g.nx * a.g.ny;
@@ -391,10 +391,9 @@ __global__ void __launch_bounds__(256) mbasis_quad_kernel(MbAnglesArgs<T> a) {
   cx<T> F[NG][kEbFields];
   double md[NG][6];
   T w1;
-  T *o = a.quad + ((uint64_t)e * P2 + quad_index(X, Y, a.g.nx, a.g.ny)) * 4;
+  T *o = a.quad + ((uint64_t)e * P2 + quad_index(X, Y, a.g.nx, a.g.ny)) * kQuadRecord;
   if (!mb_fields<T, NG>(a, X, Y, F, md, w1)) {
     o[0] = o[1] = o[2] = mb_qnan<T>();
-    o[3] = 0;
     return;
   }
   const Mat3<T> &Q = a.Q;
@@ -436,7 +435,6 @@ __global__ void __launch_bounds__(256) mbasis_quad_kernel(MbAnglesArgs<T> a) {
     o[1] = qb[0];
     o[2] = qm[0];
   }
-  o[3] = 0;
 }
 
 }  // namespace

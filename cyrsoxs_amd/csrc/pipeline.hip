@@ -693,12 +693,12 @@ cyr_status run_mbasis(const cyr_sim_desc *d, const SimGeometry &g, const void *d
           // unique inputs: the group's band values (own + mirror, each column's band
           // width, shared by its energies) and the band table; outputs: n
           // quadratic-form planes
-          kt_end("mbasis_quad", s, 2.0 * nvol * (double)wsum * cxb + 4.0 * P + n * 4.0 * P * sizeof(T));
+          kt_end("mbasis_quad", s, 2.0 * nvol * (double)wsum * cxb + 4.0 * P + n * (double)kQuadRecord * P * sizeof(T));
           kt_begin(s);
           CYR_TRY(launch_rotate_quad<T>(proj, inv, su.quad + (size_t)3 * k * g.nA, su.quad_t + (size_t)3 * k * g.nA,
                                         g.nA, g.nx, g.ny, d->rot_mask, avg, n, su.tiles, g.quarter, su.otiles,
                                         quad_pairs_share<T>(g, k), s));
-          kt_end("rotate_quad", s, n * (4.0 * P * sizeof(T) + 8.0 * P));
+          kt_end("rotate_quad", s, n * ((double)kQuadRecord * P * sizeof(T) + 8.0 * P));
           CYR_TRY(launch_detector_warp_n<T>(avg, d_out + ((uint64_t)b0 * g.nK + k) * P, (uint64_t)g.nK * P, n, g.nx,
                                             g.ny, &g.det_inv[(size_t)k * 6], d->rot_mask, s));
         }

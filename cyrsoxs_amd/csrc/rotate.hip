@@ -133,24 +133,31 @@ __device__ inline QPoint quad_point(const double *inv, uint32_t X, uint32_t Y, u
   return p;
 }
 
-template <typename T> using Quad4 = typename std::conditional<sizeof(T) == 4, float4, double4>::type;
+// One record (q0, q1, q2) of a plane of quadratic forms.
+template <typename T> struct Tap3;
+template <> struct Tap3<float> { typedef float v __attribute__((ext_vector_type(3))); };
+template <> struct Tap3<double> { typedef double v __attribute__((ext_vector_type(3))); };
+template <typename T> using Rec = typename Tap3<T>::v;
 
 // The four taps of p (clamped into the image) for NE energy planes.
 template <typename T, int NE>
-__device__ inline void quad_gather(const Quad4<T> *const (&q4)[NE], const QPoint &p, uint32_t nx, uint32_t ny,
-                                   Quad4<T> (&tp)[NE][4]) {
+__device__ inline void quad_gather(const T *const (&qp)[NE], const QPoint &p, uint32_t nx, uint32_t ny,
+                                   Rec<T> (&tp)[NE][4]) {
   const uint32_t x1 = p.x0 + 1 < nx ? p.x0 + 1 : p.x0, y1 = p.y0 + 1 < ny ? p.y0 + 1 : p.y0;
   const uint32_t off[4] = {quad_index(p.x0, p.y0, nx, ny), quad_index(x1, p.y0, nx, ny), quad_index(p.x0, y1, nx, ny),
                            quad_index(x1, y1, nx, ny)};  // P < 2^32
 #pragma unroll
   for (int e = 0; e < NE; e++)
 #pragma unroll
-    for (int i = 0; i < 4; i++) tp[e][i] = q4[e][off[i]];
+    for (int i = 0; i < 4; i++) {
+      const T *r = qp[e] + (uint64_t)off[i] * kQuadRecord;
+      tp[e][i] = Rec<T>{r[0], r[1], r[2]};
+    }
 }
 
 // Bilinear sums (q0, q1, q2) of NE energies at point p from its taps tp.
 template <typename T, int NE>
-__device__ inline void quad_sums(const QPoint &p, const Quad4<T> (&tp)[NE][4], T (&sm)[NE][3]) {
+__device__ inline void quad_sums(const QPoint &p, const Rec<T> (&tp)[NE][4], T (&sm)[NE][3]) {
   const double gx = 1 - p.fx, gy = 1 - p.fy;
   const double wd[4] = {gx * gy, p.fx * gy, gx * p.fy, p.fx * p.fy};
   // taps are interpolated in the storage precision (the weights rounded once per
@@ -168,7 +175,7 @@ __device__ inline void quad_sums(const QPoint &p, const Quad4<T> (&tp)[NE][4], T
     T s0 = 0, s1 = 0, s2 = 0;
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-      const Quad4<T> t = tp[e][i];
+      const Rec<T> t = tp[e][i];
       s0 += w[i] * (z[i] ? (T)0 : t.x);
       s1 += w[i] * (z[i] ? (T)0 : t.y);
       s2 += w[i] * (z[i] ? (T)0 : t.z);
@@ -200,7 +207,7 @@ __device__ inline void quad_combine(bool inside, const T (&sm)[NE][3], const dou
 }
 
 template <typename T, int NE>
-__device__ inline void quad_add(const QPoint &p, const Quad4<T> (&tp)[NE][4], const double *c, int rot_mask,
+__device__ inline void quad_add(const QPoint &p, const Rec<T> (&tp)[NE][4], const double *c, int rot_mask,
                                 double (&acc)[NE], uint32_t (&cnt)[NE]) {
   T sm[NE][3];
   quad_sums<T, NE>(p, tp, sm);
@@ -221,14 +228,14 @@ __device__ inline void quad_store(double *avg, uint64_t P, uint64_t t, uint32_t 
 
 // Every angle of one pixel, in angle order.
 template <typename T, int NE>
-__device__ inline void quad_pixel(const Quad4<T> *const (&q4)[NE], const double *__restrict__ inv_tab,
+__device__ inline void quad_pixel(const T *const (&qp)[NE], const double *__restrict__ inv_tab,
                                   const double *__restrict__ coef, uint32_t nA, uint32_t nx, uint32_t ny, uint32_t X,
                                   uint32_t Y, int rot_mask, double (&acc)[NE], uint32_t (&cnt)[NE]) {
 #pragma unroll 2
   for (uint32_t a = 0; a < nA; a++) {
     const QPoint p = quad_point(inv_tab + 6 * (uint64_t)a, X, Y, nx, ny);
-    Quad4<T> tp[NE][4];
-    quad_gather<T, NE>(q4, p, nx, ny, tp);
+    Rec<T> tp[NE][4];
+    quad_gather<T, NE>(qp, p, nx, ny, tp);
     quad_add<T, NE>(p, tp, coef + 3 * (uint64_t)a, rot_mask, acc, cnt);
   }
 }
@@ -244,12 +251,12 @@ __global__ void __launch_bounds__(256) rotate_quad_kernel(const T *__restrict__ 
   uint32_t X, Y;
   if (!tile_pixel(tiles, ntiles, nx, ny, X, Y)) return;
   const uint32_t e0 = blockIdx.z * NE;
-  const Quad4<T> *q4[NE];
+  const T *qp[NE];
 #pragma unroll
-  for (int e = 0; e < NE; e++) q4[e] = reinterpret_cast<const Quad4<T> *>(quad) + (uint64_t)(e0 + e < ne ? e0 + e : e0) * P;
+  for (int e = 0; e < NE; e++) qp[e] = quad + (uint64_t)(e0 + e < ne ? e0 + e : e0) * P * kQuadRecord;
   double acc[NE] = {};
   uint32_t cnt[NE] = {};
-  quad_pixel<T, NE>(q4, inv_tab, coef, nA, nx, ny, X, Y, rot_mask, acc, cnt);
+  quad_pixel<T, NE>(qp, inv_tab, coef, nA, nx, ny, X, Y, rot_mask, acc, cnt);
   quad_store<NE>(avg, P, (uint64_t)Y * nx + X, e0, ne, nA, rot_mask, acc, cnt);
 }
 
@@ -281,24 +288,21 @@ __global__ void __launch_bounds__(256) rotate_quad_kernel(const T *__restrict__ 
 // Row 0, column 0 and the centre (the tail blocks after the orbit tiles) take
 // every angle directly.
 
-template <typename T> struct Tap3;
-template <> struct Tap3<float> { typedef float v __attribute__((ext_vector_type(3))); };
-template <> struct Tap3<double> { typedef double v __attribute__((ext_vector_type(3))); };
-
 // (q0, q1, q2) of the record at byte offset `off` of a plane; offsets past the
 // buffer's size read zeros (only taps of points that take the exact path do).
 template <typename T> __device__ inline typename Tap3<T>::v tap_load(__amdgpu_buffer_rsrc_t r, uint32_t off) {
   if constexpr (sizeof(T) == 4) {
     return __builtin_amdgcn_raw_buffer_load_b96(r, off, 0, 0);
   } else {
-    typedef int i4 __attribute__((ext_vector_type(4)));
+    // 24-byte records sit at 8-byte alignment: three 8-byte loads
     typedef int i2 __attribute__((ext_vector_type(2)));
-    const i4 a = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
-    const i2 b = __builtin_amdgcn_raw_buffer_load_b64(r, off + 16, 0, 0);
+    const i2 a = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
+    const i2 b = __builtin_amdgcn_raw_buffer_load_b64(r, off + 8, 0, 0);
+    const i2 c = __builtin_amdgcn_raw_buffer_load_b64(r, off + 16, 0, 0);
     typename Tap3<double>::v v;
     v.x = __builtin_bit_cast(double, ((unsigned long long)(unsigned)a.y << 32) | (unsigned)a.x);
-    v.y = __builtin_bit_cast(double, ((unsigned long long)(unsigned)a.w << 32) | (unsigned)a.z);
-    v.z = __builtin_bit_cast(double, ((unsigned long long)(unsigned)b.y << 32) | (unsigned)b.x);
+    v.y = __builtin_bit_cast(double, ((unsigned long long)(unsigned)b.y << 32) | (unsigned)b.x);
+    v.z = __builtin_bit_cast(double, ((unsigned long long)(unsigned)c.y << 32) | (unsigned)c.x);
     return v;
   }
 }
@@ -317,9 +321,9 @@ __global__ void __launch_bounds__(256) rotate_orbit_kernel(const T *__restrict__
   const uint32_t h = n / 2;
   const uint32_t qb = xcd_tile(blockIdx.x, gridDim.x);
   const uint32_t e0 = blockIdx.z * NE;
-  const Quad4<T> *q4[NE];
+  const T *qp[NE];
 #pragma unroll
-  for (int e = 0; e < NE; e++) q4[e] = reinterpret_cast<const Quad4<T> *>(quad) + (uint64_t)(e0 + e < ne ? e0 + e : e0) * P;
+  for (int e = 0; e < NE; e++) qp[e] = quad + (uint64_t)(e0 + e < ne ? e0 + e : e0) * P * kQuadRecord;
   if (qb >= n_otiles) {
     // tail: row 0 (n pixels), column 0 below it (n - 1) and the centre
     const uint32_t j = (qb - n_otiles) * 256 + threadIdx.x;
@@ -328,7 +332,7 @@ __global__ void __launch_bounds__(256) rotate_orbit_kernel(const T *__restrict__
     const uint32_t Y = j < n ? 0 : (j < 2 * n - 1 ? j - n + 1 : h);
     double acc[NE] = {};
     uint32_t cnt[NE] = {};
-    quad_pixel<T, NE>(q4, inv_tab, coef, nA, n, n, X, Y, RM, acc, cnt);
+    quad_pixel<T, NE>(qp, inv_tab, coef, nA, n, n, X, Y, RM, acc, cnt);
     quad_store<NE>(avg, P, (uint64_t)Y * n + X, e0, ne, nA, RM, acc, cnt);
     return;
   }
@@ -343,11 +347,11 @@ __global__ void __launch_bounds__(256) rotate_orbit_kernel(const T *__restrict__
   const uint32_t PY[4] = {Y0, X0, n - Y0, n - X0};
   const double hi = (double)(n - 1);
   const double m = 1e-9 + 1e-12 * (double)n;
-  constexpr uint32_t rb = (uint32_t)sizeof(Quad4<T>);
+  constexpr uint32_t rb = kQuadRecord * (uint32_t)sizeof(T);
   __amdgpu_buffer_rsrc_t rs[NE];
 #pragma unroll
   for (int e = 0; e < NE; e++)
-    rs[e] = __builtin_amdgcn_make_buffer_rsrc((void *)q4[e], (short)0, (int)(P * rb), kRawBufferFlags);
+    rs[e] = __builtin_amdgcn_make_buffer_rsrc((void *)qp[e], (short)0, (int)(P * rb), kRawBufferFlags);
   double acc[4][NE] = {};
   uint32_t cnt[4][NE] = {};
   bool dead[4] = {false, false, false, false};
@@ -441,8 +445,8 @@ __global__ void __launch_bounds__(256) rotate_orbit_kernel(const T *__restrict__
         const uint32_t j = (i - q + 4) & 3;
         const uint32_t aa = a + q * A4;
         const QPoint d = quad_point(inv_tab + 6 * (uint64_t)aa, PX[j], PY[j], n, n);
-        Quad4<T> td[NE][4];
-        quad_gather<T, NE>(q4, d, n, n, td);
+        Rec<T> td[NE][4];
+        quad_gather<T, NE>(qp, d, n, n, td);
         T sm[NE][3];
         quad_sums<T, NE>(d, td, sm);
         const double *c = coef + 3 * (uint64_t)aa;
@@ -596,7 +600,7 @@ cyr_status launch_rotate_quad(const T *quad, const double *inv_tab, const double
   if ((uint64_t)nx * ny > 0xffffffffull) return fail(CYR_E_UNSUPPORTED, "rotation: more than 2^32 pixels");
   if (!tiles) return fail(CYR_E_INVALID, "rotation: tile order missing");
   // the orbit kernel addresses a plane through 32-bit buffer offsets
-  const bool small_plane = (uint64_t)nx * ny * sizeof(Quad4<T>) < (1ull << 31);
+  const bool small_plane = (uint64_t)nx * ny * kQuadRecord * sizeof(T) < (1ull << 31);
   const uint32_t Q = otiles && small_plane ? rotation_quarter_turns(nx, ny, nA, quarter) : 0;
   if (Q) {
     // quarter-turn orbits: each gathered tap cell serves Q (pixel, angle) pairs
@@ -617,7 +621,7 @@ cyr_status launch_rotate_quad(const T *quad, const double *inv_tab, const double
   // Energies per pass: 4 share each angle's taps; once a plane of quadratic forms
   // exceeds 16 MB (2048^2: 64 MB), one energy per pass keeps the gathers' working
   // set in cache (measured 19% faster there, equal at 512^2).
-  if ((uint64_t)nx * ny * 4 * sizeof(T) > (16ull << 20))
+  if ((uint64_t)nx * ny * kQuadRecord * sizeof(T) > (16ull << 20))
     hipLaunchKernelGGL((rotate_quad_kernel<T, 1>), dim3(xcd_grid(ntiles), 1, ne), dim3(256), 0, st, quad, inv_tab,
                        coef, nA, nx, ny, rot_mask, avg, ne, tiles, ntiles);
   else
