@@ -446,8 +446,8 @@ def main(argv=None):
             "batch": modes["mbasis_batch"]["value"], "graph": modes["mbasis_graph"]["value"],
             "batch_vs_plain": b["speedup_batch_vs_plain_serial"], "graph_vs_plain": b["speedup_graph_vs_plain_serial"],
             "batch_vs_one_at_a_time": b["speedup_batch_vs_serial"], "graph_vs_one_at_a_time": b["speedup_graph_vs_serial"],
-            "one_at_a_time_kernel_busy_fraction": modes["mbasis_serial"]["kernel_busy_fraction"],
-            "batching_bound_vs_one_at_a_time": b["batch_bound_vs_serial"],
+            "one_at_a_time_kernel_busy_fraction": modes[f"{b['pipeline']}_serial"].get("kernel_busy_fraction"),
+            "gap_removal_bound_vs_one_at_a_time": b["gap_removal_bound_vs_serial"],
             "max_rel_l2_vs_ebasis": max(modes[m]["max_rel_l2_vs_ebasis"] for m in ("mbasis_serial", "mbasis_batch",
                                                                                      "mbasis_graph")),
             "nan_masks_identical": all(modes[m]["nan_mask_identical"] for m in ("mbasis_serial", "mbasis_batch",
