@@ -179,13 +179,17 @@ __global__ void __launch_bounds__(WG) mbasis_zband_kernel(BandArgs<T> a, const c
   }
 }
 
-// Columns per tile: 32 for short pencils (256 B contiguous per z row measured
-// faster than 16 or 64 at Nz = 128), else as many as 1024 threads allow.
-constexpr int zband_pp(int N) { return N <= 128 ? 32 : (N <= 512 ? 16 : (N == 1024 ? 8 : 4)); }
+// Columns per tile: 32 for short pencils (FP32: 256 B contiguous per z row,
+// measured faster than 16 or 64 at Nz = 128, and than 16 at Nz = 256: 1024^2 x 256
+// 3.50 against 3.94 ms; FP64 keeps 16 at Nz = 256, where 32 columns would need
+// 132 KB of LDS per block), else as many as 1024 threads allow.
+template <typename T> constexpr int zband_pp(int N) {
+  return N <= 128 || (N == 256 && sizeof(T) == 4) ? 32 : (N <= 512 ? 16 : (N == 1024 ? 8 : 4));
+}
 
 template <typename T, int N>
 hipError_t launch_zband_n(const BandArgs<T> &a, hipStream_t st, const cx<T> *tw) {
-  constexpr int PP = zband_pp(N);
+  constexpr int PP = zband_pp<T>(N);
   using F = typename CtPlan<T, N, false, PP>::type;
   constexpr int WG = PP * F::TPP;
   const uint64_t tiles = (uint64_t)((a.nx + PP - 1) / PP) * a.ny * a.count;

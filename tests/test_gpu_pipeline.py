@@ -246,20 +246,37 @@ def test_mbasis_matches_ebasis(prec):
 
 
 def test_mbasis_large_plane_rotation():
-    """A 1152^2 film: quadratic-form planes above 16 MB take the one-energy-per-pass
-    rotation; held against the E-basis executor's per-angle rotation."""
-    n, M = 1152, 2
-    vox = random_euler((n, n, 1), M, 61)
+    """A 1280 x 1152 film (not square, so no quarter-turn orbits): quadratic-form
+    planes above 16 MB take the one-energy-per-pass rotation; held against the
+    E-basis executor's per-angle rotation."""
+    nx, ny, M = 1280, 1152, 2
+    assert nx * ny * 12 > 16 << 20
+    vox = random_euler((nx, ny, 1), M, 61)
     energies = [284.0, 286.0, 288.5]
     mats = random_constants(len(energies), M, 62)
     out = {}
     for pipeline in (capi.PIPE_EBASIS, capi.PIPE_MBASIS):
-        sim = capi.Sim((n, n, 1), M, 1.0, energies, (0.0, 5.0, 175.0), interpolation=1, precision=capi.FP32,
+        sim = capi.Sim((nx, ny, 1), M, 1.0, energies, (0.0, 5.0, 175.0), interpolation=1, precision=capi.FP32,
                        pipeline=pipeline)
         out[pipeline] = capi.simulate(sim, vox, mats)
     err, same = pattern_rel_l2(out[capi.PIPE_MBASIS], out[capi.PIPE_EBASIS])
     print(f"large plane: rel L2 {err:.3e}")
     assert same and err < 1e-4  # two FP32 paths, each held to 1e-4 of the oracle
+
+
+@pytest.mark.parametrize("prec", [capi.FP64, capi.FP32])
+def test_mbasis_nz256(prec):
+    """Nz = 256 pencils in the Z band pass (FP32: 32-column tiles of 1024 threads),
+    with a row width that leaves part of each tile's columns idle."""
+    dims, M = (48, 40, 256), 2
+    vox = random_euler(dims, M, 71)
+    energies = [283.0, 286.5]
+    mats = random_constants(len(energies), M, 72)
+    kw = dict(interpolation=1, windowing=1)
+    ref = _oracle("nz256", O.Sim(dims, M, 2.0, energies, (0.0, 30.0, 150.0), **kw), vox, mats)
+    sim = capi.Sim(dims, M, 2.0, energies, (0.0, 30.0, 150.0), precision=prec, pipeline=capi.PIPE_MBASIS, **kw)
+    err = _check(capi.simulate(sim, vox, mats), ref, prec)
+    print(f"nz256 prec={prec}: rel L2 {err:.3e}")
 
 
 def test_energy_lanes_bitwise():
