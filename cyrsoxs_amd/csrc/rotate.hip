@@ -357,104 +357,109 @@ __global__ void __launch_bounds__(256) rotate_orbit_kernel(const T *__restrict__
   bool dead[4] = {false, false, false, false};
   static_for<0, 4>([&](auto ic) {
     constexpr int i = decltype(ic)::value;
-    uint64_t defer = 0;  // angles (bit a) whose point takes the exact path
-    for (uint32_t a = 0; a < A4; a++) {
-      const double *inv = inv_tab + 6 * (uint64_t)a;
-      const double sx = __builtin_fma(inv[0], (double)PX[i], __builtin_fma(inv[1], (double)PY[i], inv[2]));
-      const double sy = __builtin_fma(inv[3], (double)PX[i], __builtin_fma(inv[4], (double)PY[i], inv[5]));
-      const bool inside = sx >= 0 && sx <= hi && sy >= 0 && sy <= hi;
-      const double fx0 = floor(sx), fy0 = floor(sy);
-      const double fx = sx - fx0, fy = sy - fy0;
-      const bool safe = inside ? (fx >= m && fx <= 1 - m && fy >= m && fy <= 1 - m)
-                               : (sx < -m || sx > hi + m || sy < -m || sy > hi + m);
-      const uint32_t x0 = inside ? (uint32_t)fx0 : 0u, y0 = inside ? (uint32_t)fy0 : 0u;
-      const uint32_t o00 = quad_index(x0, y0, n, n) * rb, o10 = quad_index(x0 + 1, y0, n, n) * rb;
-      const uint32_t o01 = quad_index(x0, y0 + 1, n, n) * rb, o11 = quad_index(x0 + 1, y0 + 1, n, n) * rb;
-      typename Tap3<T>::v t[NE][4];
-#pragma unroll
-      for (int e = 0; e < NE; e++) {
-        t[e][0] = tap_load<T>(rs[e], o00);
-        t[e][1] = tap_load<T>(rs[e], o10);
-        t[e][2] = tap_load<T>(rs[e], o01);
-        t[e][3] = tap_load<T>(rs[e], o11);
-      }
-      if (!safe) {
-        defer |= 1ull << a;
-        continue;
-      }
-      // every pair shares this point's cell, validity and nonzero weights
-      const T fxT = (T)fx, fyT = (T)fy, gx = (T)1 - fxT, gy = (T)1 - fyT;
-      const T w00 = gx * gy, w10 = fxT * gy, w01 = gx * fyT, w11 = fxT * fyT;
-      T sm[NE][3];
-#pragma unroll
-      for (int e = 0; e < NE; e++) {
-        sm[e][0] = w00 * t[e][0].x + w10 * t[e][1].x + w01 * t[e][2].x + w11 * t[e][3].x;
-        sm[e][1] = w00 * t[e][0].y + w10 * t[e][1].y + w01 * t[e][2].y + w11 * t[e][3].y;
-        sm[e][2] = w00 * t[e][0].z + w10 * t[e][1].z + w01 * t[e][2].z + w11 * t[e][3].z;
-      }
-      auto add = [&](auto qc, const T(&vq)[NE]) {
-        constexpr int q = decltype(qc)::value;
-        constexpr int j = (i - q + 4) & 3;  // s^q(pixel j) = pixel i
+    // angles in chunks of 64: bit (a - a0) of `defer` marks angle a of the chunk
+    // whose point takes the exact path
+    for (uint32_t a0 = 0; a0 < A4; a0 += 64) {
+      const uint32_t a1 = A4 - a0 > 64 ? a0 + 64 : A4;
+      uint64_t defer = 0;
+      for (uint32_t a = a0; a < a1; a++) {
+        const double *inv = inv_tab + 6 * (uint64_t)a;
+        const double sx = __builtin_fma(inv[0], (double)PX[i], __builtin_fma(inv[1], (double)PY[i], inv[2]));
+        const double sy = __builtin_fma(inv[3], (double)PX[i], __builtin_fma(inv[4], (double)PY[i], inv[5]));
+        const bool inside = sx >= 0 && sx <= hi && sy >= 0 && sy <= hi;
+        const double fx0 = floor(sx), fy0 = floor(sy);
+        const double fx = sx - fx0, fy = sy - fy0;
+        const bool safe = inside ? (fx >= m && fx <= 1 - m && fy >= m && fy <= 1 - m)
+                                 : (sx < -m || sx > hi + m || sy < -m || sy > hi + m);
+        const uint32_t x0 = inside ? (uint32_t)fx0 : 0u, y0 = inside ? (uint32_t)fy0 : 0u;
+        const uint32_t o00 = quad_index(x0, y0, n, n) * rb, o10 = quad_index(x0 + 1, y0, n, n) * rb;
+        const uint32_t o01 = quad_index(x0, y0 + 1, n, n) * rb, o11 = quad_index(x0 + 1, y0 + 1, n, n) * rb;
+        typename Tap3<T>::v t[NE][4];
 #pragma unroll
         for (int e = 0; e < NE; e++) {
-          if constexpr (RM) {
-            const double vv = (double)vq[e];
-            const bool ok = inside && !isnan(vv);
-            acc[j][e] += ok ? vv : 0.0;
-            cnt[j][e] += ok ? 1u : 0u;
-          } else {
-            acc[j][e] += (double)vq[e];
+          t[e][0] = tap_load<T>(rs[e], o00);
+          t[e][1] = tap_load<T>(rs[e], o10);
+          t[e][2] = tap_load<T>(rs[e], o01);
+          t[e][3] = tap_load<T>(rs[e], o11);
+        }
+        if (!safe) {
+          defer |= 1ull << (a - a0);
+          continue;
+        }
+        // every pair shares this point's cell, validity and nonzero weights
+        const T fxT = (T)fx, fyT = (T)fy, gx = (T)1 - fxT, gy = (T)1 - fyT;
+        const T w00 = gx * gy, w10 = fxT * gy, w01 = gx * fyT, w11 = fxT * fyT;
+        T sm[NE][3];
+#pragma unroll
+        for (int e = 0; e < NE; e++) {
+          sm[e][0] = w00 * t[e][0].x + w10 * t[e][1].x + w01 * t[e][2].x + w11 * t[e][3].x;
+          sm[e][1] = w00 * t[e][0].y + w10 * t[e][1].y + w01 * t[e][2].y + w11 * t[e][3].y;
+          sm[e][2] = w00 * t[e][0].z + w10 * t[e][1].z + w01 * t[e][2].z + w11 * t[e][3].z;
+        }
+        auto add = [&](auto qc, const T(&vq)[NE]) {
+          constexpr int q = decltype(qc)::value;
+          constexpr int j = (i - q + 4) & 3;  // s^q(pixel j) = pixel i
+#pragma unroll
+          for (int e = 0; e < NE; e++) {
+            if constexpr (RM) {
+              const double vv = (double)vq[e];
+              const bool ok = inside && !isnan(vv);
+              acc[j][e] += ok ? vv : 0.0;
+              cnt[j][e] += ok ? 1u : 0u;
+            } else {
+              acc[j][e] += (double)vq[e];
+            }
+          }
+          if constexpr (!RM) dead[j] = dead[j] || !inside;
+        };
+        auto value = [&](int q, T(&vq)[NE]) {
+          const T *c = coef_t + 3 * (uint64_t)(a + q * A4);
+#pragma unroll
+          for (int e = 0; e < NE; e++) vq[e] = c[0] * sm[e][0] + c[1] * sm[e][1] + c[2] * sm[e][2];
+        };
+        bool shared = false;
+        if constexpr (Q == 4) {
+          if (share) {  // pairs q and q + 2 have equal coefficients
+            T v0[NE], v1[NE];
+            value(0, v0);
+            value(1, v1);
+            add(std::integral_constant<int, 0>{}, v0);
+            add(std::integral_constant<int, 1>{}, v1);
+            add(std::integral_constant<int, 2>{}, v0);
+            add(std::integral_constant<int, 3>{}, v1);
+            shared = true;
           }
         }
-        if constexpr (!RM) dead[j] = dead[j] || !inside;
-      };
-      auto value = [&](int q, T(&vq)[NE]) {
-        const T *c = coef_t + 3 * (uint64_t)(a + q * A4);
-#pragma unroll
-        for (int e = 0; e < NE; e++) vq[e] = c[0] * sm[e][0] + c[1] * sm[e][1] + c[2] * sm[e][2];
-      };
-      bool shared = false;
-      if constexpr (Q == 4) {
-        if (share) {  // pairs q and q + 2 have equal coefficients
-          T v0[NE], v1[NE];
-          value(0, v0);
-          value(1, v1);
-          add(std::integral_constant<int, 0>{}, v0);
-          add(std::integral_constant<int, 1>{}, v1);
-          add(std::integral_constant<int, 2>{}, v0);
-          add(std::integral_constant<int, 3>{}, v1);
-          shared = true;
+        if (!shared) {
+          static_for<0, Q>([&](auto qc) {
+            T vq[NE];
+            value(decltype(qc)::value, vq);
+            add(qc, vq);
+          });
         }
       }
-      if (!shared) {
-        static_for<0, Q>([&](auto qc) {
-          T vq[NE];
-          value(decltype(qc)::value, vq);
-          add(qc, vq);
-        });
-      }
-    }
-    // points near a cell edge or the image border: every pair back-maps its own
-    // point exactly and gathers its own taps (one pair at a time, so the rare path
-    // holds few registers)
-    while (defer) {
-      const uint32_t a = (uint32_t)__builtin_ctzll(defer);
-      defer &= defer - 1;
+      // points near a cell edge or the image border: every pair back-maps its own
+      // point exactly and gathers its own taps (one pair at a time, so the rare path
+      // holds few registers)
+      while (defer) {
+        const uint32_t a = a0 + (uint32_t)__builtin_ctzll(defer);
+        defer &= defer - 1;
 #pragma unroll 1
-      for (uint32_t q = 0; q < (uint32_t)Q; q++) {
-        const uint32_t j = (i - q + 4) & 3;
-        const uint32_t aa = a + q * A4;
-        const QPoint d = quad_point(inv_tab + 6 * (uint64_t)aa, PX[j], PY[j], n, n);
-        Rec<T> td[NE][4];
-        quad_gather<T, NE>(qp, d, n, n, td);
-        T sm[NE][3];
-        quad_sums<T, NE>(d, td, sm);
-        const double *c = coef + 3 * (uint64_t)aa;
-        switch (j) {  // constant indices keep the accumulators in registers
-          case 0: quad_combine<T, NE>(d.inside, sm, c, RM, acc[0], cnt[0]); break;
-          case 1: quad_combine<T, NE>(d.inside, sm, c, RM, acc[1], cnt[1]); break;
-          case 2: quad_combine<T, NE>(d.inside, sm, c, RM, acc[2], cnt[2]); break;
-          default: quad_combine<T, NE>(d.inside, sm, c, RM, acc[3], cnt[3]); break;
+        for (uint32_t q = 0; q < (uint32_t)Q; q++) {
+          const uint32_t j = (i - q + 4) & 3;
+          const uint32_t aa = a + q * A4;
+          const QPoint d = quad_point(inv_tab + 6 * (uint64_t)aa, PX[j], PY[j], n, n);
+          Rec<T> td[NE][4];
+          quad_gather<T, NE>(qp, d, n, n, td);
+          T sm[NE][3];
+          quad_sums<T, NE>(d, td, sm);
+          const double *c = coef + 3 * (uint64_t)aa;
+          switch (j) {  // constant indices keep the accumulators in registers
+            case 0: quad_combine<T, NE>(d.inside, sm, c, RM, acc[0], cnt[0]); break;
+            case 1: quad_combine<T, NE>(d.inside, sm, c, RM, acc[1], cnt[1]); break;
+            case 2: quad_combine<T, NE>(d.inside, sm, c, RM, acc[2], cnt[2]); break;
+            default: quad_combine<T, NE>(d.inside, sm, c, RM, acc[3], cnt[3]); break;
+          }
         }
       }
     }

@@ -102,6 +102,9 @@ ORBIT_CASES = {
     "q3_nn_rotmask": dict(dims=(40, 40, 8), angles=(0.0, 30.0, 240.0), kw=dict(interpolation=0, rot_mask=1)),
     "q4_thin_linear": dict(dims=(64, 64, 1), angles=(0.0, 10.0, 350.0), kw=dict(interpolation=1)),
     "q4_offset_rotmask": dict(dims=(32, 32, 8), angles=(7.0, 18.0, 349.0), kw=dict(interpolation=1, rot_mask=1)),
+    # more than 64 angles per quarter turn (the exact-path record spans several words)
+    "q4_half_deg": dict(dims=(32, 32, 1), angles=(0.0, 0.5, 359.5), kw=dict(interpolation=1)),
+    "q4_fine_rotmask": dict(dims=(24, 24, 4), angles=(0.0, 1.25, 358.75), kw=dict(interpolation=1, rot_mask=1)),
 }
 
 
