@@ -69,6 +69,11 @@ CASES = {
     "deep_z_nn": dict(dims=(24, 32, 64), M=2, morph=1, kw=dict(interpolation=0)),
     # run-time X plan (no compile-time codelet for 288) with rows wider than a block
     "rt_wide_rows": dict(dims=(288, 288, 4), M=3, morph=0, kw=dict(interpolation=1, windowing=1)),
+    # 720 angles: square (quarter-turn orbits, 180 steps each) and not square
+    "many_angles": dict(dims=(24, 24, 8), M=2, morph=0, kw=dict(interpolation=1, windowing=1),
+                        angles=(0.0, 0.5, 359.5)),
+    "many_angles_rect": dict(dims=(40, 24, 1), M=2, morph=1, kw=dict(interpolation=1, rot_mask=1),
+                             angles=(0.0, 0.5, 359.5)),
 }
 
 
@@ -82,7 +87,7 @@ def test_modes(case, prec, pipeline):
     vox = random_euler(dims, M, 21) if cfg["morph"] == 0 else random_vector(dims, M, 21)
     energies = [282.0, 285.5]
     mats = random_constants(len(energies), M, 4)
-    angles = (0.0, 20.0, 160.0)
+    angles = cfg.get("angles", (0.0, 20.0, 160.0))
     kw = dict(cfg["kw"], morphology_type=cfg["morph"])
     phys = 1.0 if dims[2] == 1 else 5.0
     osim = O.Sim(dims, M, phys, energies, angles, **kw)
