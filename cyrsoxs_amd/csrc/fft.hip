@@ -160,6 +160,19 @@ __global__ void __launch_bounds__(1024) fft_pencil_kernel(StagePlan P, PassArgs 
 // are padded LDS rows, several per wave when short, twiddles read from the
 // global table; strided pencils (Y, Z) are PP-column tiles whose twiddles are
 // staged in LDS next to the tile once per block.
+// Load of one complex element marked non-temporal (streamed: each element of a
+// strided pass is read once).  For the 512-point Y pass over 7 volumes of
+// 512^2 x 128 it cuts the pass from 717 to 637 us; non-temporal stores add nothing.
+template <typename T> __device__ inline cx<T> stream_load(const cx<T> *p) {
+  if constexpr (sizeof(T) == 4) {
+    const unsigned long long v = __builtin_nontemporal_load(reinterpret_cast<const unsigned long long *>(p));
+    return mk<T>(__uint_as_float((unsigned)v), __uint_as_float((unsigned)(v >> 32)));
+  } else {
+    const double *d = reinterpret_cast<const double *>(p);
+    return mk<T>(__builtin_nontemporal_load(d), __builtin_nontemporal_load(d + 1));
+  }
+}
+
 template <typename T, class F, bool CONTIG, int WG>
 __global__ void __launch_bounds__(WG) fft_ct_kernel(PassArgs A, const cx<T> *__restrict__ tw) {
   constexpr int E = F::kE, TPP = F::TPP, N = TPP * E;
@@ -193,7 +206,10 @@ __global__ void __launch_bounds__(WG) fft_ct_kernel(PassArgs A, const cx<T> *__r
                  (uint64_t)o * A.outer_stride;
   cx<T> v[E];
 #pragma unroll
-  for (int i = 0; i < E; i++) v[i] = active ? gbase[(uint64_t)F::in_pos(t, i) * A.elem_stride] : mk<T>(0, 0);
+  for (int i = 0; i < E; i++) {
+    if constexpr (CONTIG) v[i] = active ? gbase[(uint64_t)F::in_pos(t, i) * A.elem_stride] : mk<T>(0, 0);
+    else v[i] = active ? stream_load<T>(gbase + (uint64_t)F::in_pos(t, i) * A.elem_stride) : mk<T>(0, 0);
+  }
   if constexpr (!CONTIG) {
     // twiddles staged while the pencil loads are in flight, with the last
     // stage's FP32 residuals when they fit without costing occupancy (else they
