@@ -19,6 +19,7 @@ choice between the E-basis and material-basis executors per request) on one
 slot (FIFO, no overlap) at one rate, for comparison.
 
     python -m cyrsoxs_amd.bench_serving [--rates 2,5,10,20,40] [--window 15] [--slots 3] [--pipeline auto|mbasis|ebasis]
+                                         [--fifo-rate R (0: none)] [--plain-window S (0: none)]
 """
 from __future__ import annotations
 
@@ -114,6 +115,7 @@ def run(a):
         ids, units = [], {}
         with capi.Engine(slots=slots) as eng:
             t_start = time.perf_counter()
+            t_note = t_start
             for at, (sim, vox, mats, u) in zip(arrivals, reqs):
                 now = time.perf_counter() - t_start
                 if at > now:
@@ -121,6 +123,11 @@ def run(a):
                 rid = eng.submit(sim, vox, mats)
                 ids.append(rid)
                 units[rid] = u
+                if time.perf_counter() - t_note >= 60.0:  # progress of long windows
+                    t_note = time.perf_counter()
+                    st = eng.stats()
+                    print(f"rate {rate:g}: {t_note - t_start:.0f} s, {st['submitted']} submitted, "
+                          f"{st['completed']} completed, {st['queued']} queued", file=sys.stderr, flush=True)
             deadline = time.perf_counter() + a.drain
             for rid in ids:
                 left = deadline - time.perf_counter()
@@ -166,11 +173,15 @@ def run(a):
         results.append(res)
     # the same request stream as the continuous-batching run at that rate (or a
     # fresh one if that rate was not run), served FIFO on one slot
-    fi = rates.index(a.fifo_rate) if a.fifo_rate in rates else 77
-    fifo = serve(a.fifo_rate, a.window, 1, PIPE, a.seed * 1000 + fi)
-    print(json.dumps(fifo), file=sys.stderr, flush=True)
-    plain = serve(min(rates), a.plain_window, 1, capi.PIPE_PLAIN, a.seed * 1000 + rates.index(min(rates)))
-    print(json.dumps(plain), file=sys.stderr, flush=True)
+    # (a rate or window of 0 leaves the comparison out)
+    fifo = plain = None
+    if a.fifo_rate > 0:
+        fi = rates.index(a.fifo_rate) if a.fifo_rate in rates else 77
+        fifo = serve(a.fifo_rate, a.window, 1, PIPE, a.seed * 1000 + fi)
+        print(json.dumps(fifo), file=sys.stderr, flush=True)
+    if a.plain_window > 0:
+        plain = serve(min(rates), a.plain_window, 1, capi.PIPE_PLAIN, a.seed * 1000 + rates.index(min(rates)))
+        print(json.dumps(plain), file=sys.stderr, flush=True)
     out = {"metric": "p50/p99 latency and voxel·angle·energy/s under a Poisson request stream",
            "workload": "grids " + a.grids + ", 2-4 materials, 1-50 energies 280-305 eV, EAngle 0:5:175, linear, "
                        "Hanning, FP32",

@@ -279,9 +279,10 @@ __global__ void __launch_bounds__(256) rotate_quad_kernel(const T *__restrict__ 
 // the lean path: no zero-weight tests, taps through 32-bit buffer offsets (the
 // plane is below 2 GiB), the intensity in the compute precision, and one value
 // for the pairs q and q + 2 when their coefficients are equal (`share`).  The
-// rest (a point on or near a pixel edge: angles at exact multiples of 90 degrees)
-// is recorded per angle and replayed after the main loop: there each pair
-// back-maps its own point and gathers its own taps, as quad_pixel does.
+// rest (a point on or near a pixel edge: angles at exact multiples of 90 degrees,
+// or special angles such as 60 degrees on the axes) is recorded per angle in a
+// 64-bit mask and replayed after each run of 64 angles: there each pair back-maps
+// its own point and gathers its own taps, as quad_pixel does.
 // Validity and NaN taps are therefore decided exactly as per pair; only the order
 // of the FP64 sums over angles differs.  Without the rotation mask a point
 // outside the image marks its pairs' pixels NaN (the value the sum would reach).
