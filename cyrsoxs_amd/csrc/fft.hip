@@ -42,6 +42,9 @@ struct PassArgs {
   uint32_t groups;  // ceil(inner_count / pp)
   uint32_t pp;
   uint32_t tpp;
+  double2 *dcz;        // non-null: the Y pass adds the DC-neighbour partial sums (FftDc)
+  const void *dc_twz;  // Z twiddles of those sums (null in 2D)
+  uint32_t dc_nz;
 };
 
 // ---- one Stockham stage ------------------------------------------------------
@@ -209,6 +212,34 @@ __global__ void __launch_bounds__(WG) fft_ct_kernel(PassArgs A, const cx<T> *__r
   for (int i = 0; i < E; i++) {
     if constexpr (CONTIG) v[i] = active ? gbase[(uint64_t)F::in_pos(t, i) * A.elem_stride] : mk<T>(0, 0);
     else v[i] = active ? stream_load<T>(gbase + (uint64_t)F::in_pos(t, i) * A.elem_stride) : mk<T>(0, 0);
+  }
+  if constexpr (!CONTIG) {
+    // DC-neighbour sums (Y pass of X-transformed volumes, Nx >= 3): columns 0, 1
+    // and Nx - 1 of slice z = o add sum_y w G(x, y, z), w = 2 cos(2 pi y / Ny)
+    // + 2 cos(2 pi z / Nz) (3D) at x = 0 and 1 otherwise; one FP64 partial per
+    // (volume, z, column, wave), the lanes of a wave that hold one column (lane
+    // stride pp) combined in a fixed order
+    if (A.dcz && (g == 0 || g == A.groups - 1)) {  // the same for the whole block
+      const bool mine = active && (pencil <= 1 || pencil == A.inner_count - 1);
+      const uint32_t which = pencil <= 1 ? pencil : 2;
+      double sr = 0, si = 0;
+      if (mine) {
+        const double cz =
+            A.dc_nz > 1 ? 2.0 * tw_cos<T>(reinterpret_cast<const cx<T> *>(A.dc_twz), A.dc_nz, o) : 0.0;
+#pragma unroll
+        for (int i = 0; i < E; i++) {
+          const double w = which == 0 ? 2.0 * tw_cos<T>(tw, N, F::in_pos(t, i)) + cz : 1.0;
+          sr += w * (double)v[i].x;
+          si += w * (double)v[i].y;
+        }
+      }
+      for (uint32_t off = A.pp; off < 64; off <<= 1) {
+        sr += __shfl_xor(sr, (int)off);
+        si += __shfl_xor(si, (int)off);
+      }
+      if (mine && (threadIdx.x & 63) < A.pp)
+        A.dcz[(((uint64_t)vol * A.dc_nz + o) * 3 + which) * (WG / 64) + threadIdx.x / 64] = make_double2(sr, si);
+    }
   }
   if constexpr (!CONTIG) {
     // twiddles staged while the pencil loads are in flight, with the last
@@ -487,9 +518,27 @@ void fft_release() {
   g_tw.clear();
 }
 
+// Whether a strided pass of length n runs the compile-time codelets.
+template <typename T> bool strided_ct(uint32_t n) {
+  if (!ct_length(n) || (sizeof(T) == 8 && ct_elements(n) > 8)) return false;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (g_shapes.count(std::make_tuple(n, 0, prec_of<T>()))) return false;
+  }
+  const uint32_t pp = n <= 512 ? 16 : (n == 1024 ? 8 : 4);
+  return ((size_t)pp * n + n) * sizeof(cx<T>) <= max_lds_bytes();
+}
+
+// (the strided codelet's block: pp pencils x n / E threads, in waves of 64)
+template <typename T> uint32_t fft_dc_slots(uint32_t nx, uint32_t ny) {
+  if (nx < 3 || !strided_ct<T>(ny)) return 0;
+  const uint32_t pp = ny <= 512 ? 16 : (ny == 1024 ? 8 : 4);
+  return 3 * (pp * (ny / ct_elements(ny)) / 64);
+}
+
 template <typename T>
 cyr_status fft_axis(void *const *vols, uint32_t count, uint32_t nx, uint32_t ny, uint32_t nz, int axis,
-                    hipStream_t stream) {
+                    hipStream_t stream, const FftDc *dc) {
   const uint32_t n = axis == 0 ? nx : axis == 1 ? ny : nz;
   if (n <= 1 || count == 0) return CYR_OK;
   cyr_status st;
@@ -519,21 +568,31 @@ cyr_status fft_axis(void *const *vols, uint32_t count, uint32_t nx, uint32_t ny,
   const Factoring f = factor(n);
   const size_t lds_max = max_lds_bytes();
   // compile-time codelets unless a shape override asks for the run-time plan
-  bool use_ct = ct_length(n) && (sizeof(T) == 4 || ct_elements(n) <= 8);
-  {
-    std::lock_guard<std::mutex> lk(g_mu);
-    if (g_shapes.count(std::make_tuple(n, (int)contig, prec_of<T>()))) use_ct = false;
+  bool use_ct;
+  if (contig) {
+    use_ct = ct_length(n) && (sizeof(T) == 4 || ct_elements(n) <= 8);
+    {
+      std::lock_guard<std::mutex> lk(g_mu);
+      if (g_shapes.count(std::make_tuple(n, 1, prec_of<T>()))) use_ct = false;
+    }
+    if (use_ct) {
+      const size_t need = (size_t)(64 / (n / ct_elements(n)) > 1 ? 64 / (n / ct_elements(n)) : 1) *
+                          ct_row_pitch(n) * sizeof(cx<T>);
+      if (need > lds_max) use_ct = false;
+    }
+  } else {
+    use_ct = strided_ct<T>(n);
   }
-  if (use_ct) {
-    const uint32_t pp = contig ? 1 : (n <= 512 ? 16 : (n == 1024 ? 8 : 4));
-    const size_t need = contig ? (size_t)(64 / (n / ct_elements(n)) > 1 ? 64 / (n / ct_elements(n)) : 1) *
-                                     ct_row_pitch(n) * sizeof(cx<T>)
-                               : ((size_t)pp * n + n) * sizeof(cx<T>);
-    if (need > lds_max) use_ct = false;
-  }
+  // DC-neighbour sums only ride along a Y pass on the compile-time codelets
+  const uint64_t dc_per_vol = dc ? (uint64_t)nz * fft_dc_slots<T>(nx, ny) : 0;
+  if (dc && (axis != 1 || !use_ct || dc_per_vol == 0))
+    return fail(CYR_E_INVALID, "FFT: DC sums need a Y pass on a compile-time plan and Nx >= 3");
   for (uint32_t c0 = 0; c0 < count; c0 += 8) {
     const uint32_t nc = count - c0 < 8 ? count - c0 : 8;
     for (uint32_t i = 0; i < nc; i++) A.vol[i] = vols[c0 + i];
+    A.dcz = dc ? dc->dcz + c0 * dc_per_vol : nullptr;
+    A.dc_twz = dc ? dc->twz : nullptr;
+    A.dc_nz = nz;
     bool done = false;
     if (use_ct) {
       bool found = false;
@@ -613,8 +672,12 @@ cyr_status fft3d(void *const *vols, uint32_t count, uint32_t nx, uint32_t ny, ui
 
 template const cx<float> *twiddles<float>(uint32_t, cyr_status *);
 template const cx<double> *twiddles<double>(uint32_t, cyr_status *);
-template cyr_status fft_axis<float>(void *const *, uint32_t, uint32_t, uint32_t, uint32_t, int, hipStream_t);
-template cyr_status fft_axis<double>(void *const *, uint32_t, uint32_t, uint32_t, uint32_t, int, hipStream_t);
+template cyr_status fft_axis<float>(void *const *, uint32_t, uint32_t, uint32_t, uint32_t, int, hipStream_t,
+                                    const FftDc *);
+template cyr_status fft_axis<double>(void *const *, uint32_t, uint32_t, uint32_t, uint32_t, int, hipStream_t,
+                                     const FftDc *);
+template uint32_t fft_dc_slots<float>(uint32_t, uint32_t);
+template uint32_t fft_dc_slots<double>(uint32_t, uint32_t);
 template cyr_status fft3d<float>(void *const *, uint32_t, uint32_t, uint32_t, uint32_t, hipStream_t);
 template cyr_status fft3d<double>(void *const *, uint32_t, uint32_t, uint32_t, uint32_t, hipStream_t);
 

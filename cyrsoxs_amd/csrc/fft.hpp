@@ -9,11 +9,25 @@
 
 namespace cyr {
 
+// DC-neighbour sums of X-transformed volumes taken by their Y pass (see
+// fft_axis): dcz receives count x nz x fft_dc_slots(nx, ny) FP64 partials, in
+// [volume][z][slot] order, whose fixed-order sum over one volume is
+//   sum_{y,z} G(1,y,z) + G(Nx-1,y,z) + G(0,y,z) (2 cos(2 pi y/Ny) + 2 cos(2 pi z/Nz)),
+// the z term only in 3D (twz: the Z twiddle table, null in 2D).
+struct FftDc {
+  double2 *dcz;
+  const void *twz;
+};
+// Partials per (volume, z) of a Y pass that can take the DC sums; 0 when it
+// cannot (Nx < 3, or no compile-time plan for Ny).
+template <typename T> uint32_t fft_dc_slots(uint32_t nx, uint32_t ny);
+
 // One axis pass over `count` volumes laid out X-fastest with dims (nx, ny, nz).
-// axis 0 = X (contiguous pencils), 1 = Y, 2 = Z (strided pencils).
+// axis 0 = X (contiguous pencils), 1 = Y, 2 = Z (strided pencils).  With dc
+// (axis 1 only, fft_dc_slots > 0) the pass also writes the DC-neighbour partials.
 template <typename T>
 cyr_status fft_axis(void *const *vols, uint32_t count, uint32_t nx, uint32_t ny, uint32_t nz,
-                    int axis, hipStream_t stream);
+                    int axis, hipStream_t stream, const FftDc *dc = nullptr);
 
 // Full forward unnormalised 3D FFT (X, then Y, then Z) of `count` volumes.
 template <typename T>

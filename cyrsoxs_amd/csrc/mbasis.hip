@@ -201,9 +201,9 @@ hipError_t launch_zband_n(const BandArgs<T> &a, hipStream_t st, const cx<T> *tw)
 }
 
 // dcv[v] = (sum_z dcz[v][z]) / cnt, fixed-order tree (deterministic); one block per volume
-__global__ void __launch_bounds__(256) mbasis_dcsum_kernel(const double2 *__restrict__ dcz, uint32_t nz, double cnt,
-                                                           double2 *__restrict__ dcv) {
-  __shared__ double red[2][256];
+__global__ void __launch_bounds__(1024) mbasis_dcsum_kernel(const double2 *__restrict__ dcz, uint32_t nz, double cnt,
+                                                            double2 *__restrict__ dcv) {
+  __shared__ double red[2][1024];
   const uint32_t v = blockIdx.x;
   double sr = 0, si = 0;
   for (uint32_t z = threadIdx.x; z < nz; z += blockDim.x) {
@@ -468,7 +468,7 @@ template <typename T> cyr_status launch_mbasis_extract(const BandArgs<T> &a, hip
 cyr_status launch_mbasis_dcsum(const double2 *dcz, uint32_t nvol, uint32_t nz, bool three, double2 *dcv,
                                hipStream_t st) {
   if (nvol == 0) return CYR_OK;
-  hipLaunchKernelGGL(mbasis_dcsum_kernel, dim3(nvol), dim3(256), 0, st, dcz, nz, three ? 6.0 : 4.0, dcv);
+  hipLaunchKernelGGL(mbasis_dcsum_kernel, dim3(nvol), dim3(1024), 0, st, dcz, nz, three ? 6.0 : 4.0, dcv);
   CYR_LAUNCHED();
   return CYR_OK;
 }

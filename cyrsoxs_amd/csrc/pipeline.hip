@@ -565,12 +565,16 @@ cyr_status run_mbasis(const cyr_sim_desc *d, const SimGeometry &g, const void *d
   CYR_TRY(ensure_buffer(lb.proj, sizeof(T) * P * (lab ? 4 * (uint64_t)nb : g.nA)));
   CYR_TRY(ensure_buffer(lb.avg, sizeof(double) * P * nb));
   CYR_TRY(ensure_buffer(lb.bins, sizeof(int32_t) * P + sizeof(uint32_t) * mbasis_bmax_words(g.nx, g.ny)));
-  CYR_TRY(ensure_buffer(lb.dc, sizeof(double2) * nvol * (g.nz + 1)));
+  // DC-neighbour sums: taken by the Y pass when it runs on a compile-time plan
+  // (dc_slots partials per z slice), else by their own kernel (one per slice)
+  const uint32_t dc_slots = fft_dc_slots<T>(g.nx, g.ny);
+  const uint64_t dc_per = (uint64_t)g.nz * (dc_slots ? dc_slots : 1);
+  CYR_TRY(ensure_buffer(lb.dc, sizeof(double2) * nvol * (dc_per + 1)));
   cx<T> *pol = reinterpret_cast<cx<T> *>(lb.pol.ptr);
   int32_t *band = reinterpret_cast<int32_t *>(lb.bins.ptr);
   uint32_t *bmax = reinterpret_cast<uint32_t *>(band + P);
   double2 *dcz = reinterpret_cast<double2 *>(lb.dc.ptr);
-  double2 *dcv = dcz + (uint64_t)nvol * g.nz;
+  double2 *dcv = dcz + (uint64_t)nvol * dc_per;
   T *proj = reinterpret_cast<T *>(lb.proj.ptr);
   double *avg = reinterpret_cast<double *>(lb.avg.ptr);
   const double cxb = su.cxb;
@@ -618,20 +622,23 @@ cyr_status run_mbasis(const cyr_sim_desc *d, const SimGeometry &g, const void *d
         kt_begin(s);
         CYR_TRY(launch_mbasis_xfields<T>(xa, pair, d_vox, d->storage, d->morphology_type, hann, s));
         kt_end("xfields", s, (pair ? 2.0 : 1.0) * 4.0 * vel * V + nv * (double)V * cxb);
-        DcArgs<T> da{};
-        for (uint32_t c = 0; c < nv; c++) da.vol[c] = xa.out[c];
-        da.count = nv;
-        da.twy = su.twy;
-        da.twz = su.twz;
-        da.nx = g.nx;
-        da.ny = g.ny;
-        da.nz = g.nz;
-        da.dcz = dcz + (uint64_t)vb * g.nz;
+        FftDc fdc{dcz + (uint64_t)vb * dc_per, su.twz};
+        if (!dc_slots) {
+          DcArgs<T> da{};
+          for (uint32_t c = 0; c < nv; c++) da.vol[c] = xa.out[c];
+          da.count = nv;
+          da.twy = su.twy;
+          da.twz = su.twz;
+          da.nx = g.nx;
+          da.ny = g.ny;
+          da.nz = g.nz;
+          da.dcz = fdc.dcz;
+          kt_begin(s);
+          CYR_TRY(launch_ebasis_dc<T>(da, s));
+          kt_end("ebasis_dc", s, 3.0 * nv * (double)g.ny * g.nz * cxb);
+        }
         kt_begin(s);
-        CYR_TRY(launch_ebasis_dc<T>(da, s));
-        kt_end("ebasis_dc", s, 3.0 * nv * (double)g.ny * g.nz * cxb);
-        kt_begin(s);
-        CYR_TRY(fft_axis<T>(vptr, nv, g.nx, g.ny, g.nz, 1, s));
+        CYR_TRY(fft_axis<T>(vptr, nv, g.nx, g.ny, g.nz, 1, s, dc_slots ? &fdc : nullptr));
         kt_end("fft_y", s, 2.0 * nv * (double)V * cxb);
         BandArgs<T> ba{};
         for (uint32_t c = 0; c < nv; c++) ba.vol[c] = xa.out[c];
@@ -665,7 +672,7 @@ cyr_status run_mbasis(const cyr_sim_desc *d, const SimGeometry &g, const void *d
         m0 += pair ? 2 : 1;
         vb += nv;
       }
-      CYR_TRY(launch_mbasis_dcsum(dcz, nvol, g.nz, three, dcv, s));
+      CYR_TRY(launch_mbasis_dcsum(dcz, nvol, (uint32_t)dc_per, three, dcv, s));
       // ---- per energy: fields at the Ewald bins, angles, rotation
       const double kv[3] = {g.kvec[k].x, g.kvec[k].y, g.kvec[k].z};
       const double *inv = su.rotinv + (size_t)6 * k * g.nA;
@@ -868,6 +875,12 @@ uint64_t output_bytes(const cyr_sim_desc *d, const SimGeometry &g) {
   return (uint64_t)g.nE * g.nK * g.P * (d->precision == CYR_FP64 ? 8 : 4);
 }
 
+// DC-neighbour partials per volume of the material basis (see run_mbasis)
+static uint64_t mbasis_dc_per(const cyr_sim_desc *d, const SimGeometry &g) {
+  const uint32_t slots = d->precision == CYR_FP64 ? fft_dc_slots<double>(g.nx, g.ny) : fft_dc_slots<float>(g.nx, g.ny);
+  return (uint64_t)g.nz * (slots ? slots : 1);
+}
+
 uint64_t work_bytes(const cyr_sim_desc *d, const SimGeometry &g) {
   const uint64_t cs = d->precision == CYR_FP64 ? 16 : 8, rs = cs / 2;
   const int pipe = resolved_pipeline(d, g);
@@ -880,7 +893,7 @@ uint64_t work_bytes(const cyr_sim_desc *d, const SimGeometry &g) {
     const uint64_t nb = d->reference_frame != 0 ? std::min(mbasis_energy_batch(g, rs), g.nE) : 1;
     const uint64_t planes = d->reference_frame != 0 ? 4 * nb : g.nA;
     b += cs * g.V * pass + cs * 2 * nv * J * g.P + 4 * g.P + 4 * mbasis_bmax_words(g.nx, g.ny) +
-         16 * nv * (g.nz + 1) + rs * g.P * planes + 8 * g.P * nb;
+         16 * nv * (mbasis_dc_per(d, g) + 1) + rs * g.P * planes + 8 * g.P * nb;
   } else if (pipe == CYR_PIPE_EBASIS) {  // per lane: fields, bin table, Y/z partial sums (chunks x 2 bins x fields), DC sums,
                  // per-angle projections, average
     const uint64_t nch = ebasis_max_chunks(g.nx, g.ny, g.nz, d->precision);
