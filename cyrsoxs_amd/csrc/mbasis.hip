@@ -380,27 +380,8 @@ __global__ void __launch_bounds__(256) mbasis_angles_kernel(MbAnglesArgs<T> a) {
 // v = Q (F1, F3, F4), so I(theta) = c^2 |M u|^2 + s^2 |M v|^2 + 2 c s Re<M u, M v>.
 // quad[pixel] = (|Mu|^2, |Mv|^2, Re<Mu, Mv>), interpolated between the bins.
 template <typename T, int NG>
-__global__ void __launch_bounds__(256) mbasis_quad_kernel(MbAnglesArgs<T> a) {
-  const uint64_t P2 = (uint64_t)a.g.nx * a.g.ny;
-  // 1D grid in XCD order over (pixel block, energy), energy fastest: the ne
-  // energies of a pixel block read the same band columns and run on one XCD
-  // back to back, so all but the first find them in its L2
-  const uint32_t q = xcd_tile(blockIdx.x, gridDim.x);
-  const uint32_t e = q % a.ne, b = q / a.ne;
-  const uint64_t t = (uint64_t)b * blockDim.x + threadIdx.x;
-  if (t >= P2) return;
-  a.g.k = a.kmag[e];
-  a.mats += (uint64_t)e * a.M;
-  const uint32_t Y = (uint32_t)(t / a.g.nx), X = (uint32_t)(t - (uint64_t)Y * a.g.nx);
-  cx<T> F[NG][kEbFields];
-  double md[NG][6];
-  T w1;
-  T *o = a.quad + ((uint64_t)e * P2 + quad_index(X, Y, a.g.nx, a.g.ny)) * kQuadRecord;
-  if (!mb_fields<T, NG>(a, X, Y, F, md, w1)) {
-    o[0] = o[1] = o[2] = mb_qnan<T>();
-    return;
-  }
-  const Mat3<T> &Q = a.Q;
+__device__ inline void quad_form_write(const Mat3<T> &Q, const cx<T> (&F)[NG][kEbFields], const double (&md)[NG][6],
+                                       T w1, T *o) {
   T qa[NG], qb[NG], qm[NG];
 #pragma unroll
   for (int q = 0; q < NG; q++) {
@@ -438,6 +419,151 @@ __global__ void __launch_bounds__(256) mbasis_quad_kernel(MbAnglesArgs<T> a) {
     o[0] = qa[0];
     o[1] = qb[0];
     o[2] = qm[0];
+  }
+}
+
+// One (pixel, energy): a holds the energy's |k| and constants.
+template <typename T, int NG>
+__device__ inline void quad_pixel_energy(const MbAnglesArgs<T> &a, uint32_t X, uint32_t Y, T *o) {
+  cx<T> F[NG][kEbFields];
+  double md[NG][6];
+  T w1;
+  if (!mb_fields<T, NG>(a, X, Y, F, md, w1)) {
+    o[0] = o[1] = o[2] = mb_qnan<T>();
+    return;
+  }
+  quad_form_write<T, NG>(a.Q, F, md, w1, o);
+}
+
+template <typename T, int NG>
+__global__ void __launch_bounds__(256) mbasis_quad_kernel(MbAnglesArgs<T> a) {
+  const uint64_t P2 = (uint64_t)a.g.nx * a.g.ny;
+  // 1D grid in XCD order over (pixel block, energy), energy fastest: the ne
+  // energies of a pixel block read the same band columns and run on one XCD
+  // back to back, so all but the first find them in its L2
+  const uint32_t q = xcd_tile(blockIdx.x, gridDim.x);
+  const uint32_t e = q % a.ne, b = q / a.ne;
+  const uint64_t t = (uint64_t)b * blockDim.x + threadIdx.x;
+  if (t >= P2) return;
+  a.g.k = a.kmag[e];
+  a.mats += (uint64_t)e * a.M;
+  const uint32_t Y = (uint32_t)(t / a.g.nx), X = (uint32_t)(t - (uint64_t)Y * a.g.nx);
+  quad_pixel_energy<T, NG>(a, X, Y, a.quad + ((uint64_t)e * P2 + quad_index(X, Y, a.g.nx, a.g.ny)) * kQuadRecord);
+}
+
+// Two energies (2 ep, 2 ep + 1) of one pixel per thread.  Neighbouring energies'
+// first Ewald bins are usually equal or one apart; then every band value either
+// energy needs lies in the NG + 1 bins from the lower one, so each volume's values
+// are gathered (and unpacked) once for both: the kernel is bound by its gathers.
+// Pixels whose bins lie further apart, the FFT bin (0, 0) (DC values) and NaN
+// pixels take the one-energy path for each energy.
+template <typename T, int NG>
+__global__ void __launch_bounds__(256) mbasis_quad2_kernel(MbAnglesArgs<T> a) {
+  const uint64_t P2 = (uint64_t)a.g.nx * a.g.ny;
+  const uint32_t npair = (a.ne + 1) / 2;
+  const uint32_t qt = xcd_tile(blockIdx.x, gridDim.x);
+  const uint32_t ep = qt % npair, b = qt / npair;
+  const uint64_t t = (uint64_t)b * blockDim.x + threadIdx.x;
+  if (t >= P2) return;
+  const uint32_t e0 = 2 * ep;
+  const bool two = e0 + 1 < a.ne;
+  const uint32_t Y = (uint32_t)(t / a.g.nx), X = (uint32_t)(t - (uint64_t)Y * a.g.nx);
+  const uint32_t rec = quad_index(X, Y, a.g.nx, a.g.ny);
+  MbAnglesArgs<T> ak[2] = {a, a};
+#pragma unroll
+  for (int k = 0; k < 2; k++) {
+    const uint32_t e = two ? e0 + k : e0;
+    ak[k].g.k = a.kmag[e];
+    ak[k].mats += (uint64_t)e * a.M;
+  }
+  T *o0 = a.quad + ((uint64_t)e0 * P2 + rec) * kQuadRecord, *o1 = o0 + P2 * kQuadRecord;
+  uint32_t z0[2];
+  double f[2];
+  const int kind0 = ewald_geometry(ak[0].g, X, Y, &z0[0], &f[0]);
+  const int kind1 = two ? ewald_geometry(ak[1].g, X, Y, &z0[1], &f[1]) : 0;
+  const uint32_t ix = unshift(X, a.g.nx), iy = unshift(Y, a.g.ny);
+  const bool three = !a.g.enable2d;
+  const uint32_t gap = z0[0] > z0[1] ? z0[0] - z0[1] : z0[1] - z0[0];
+  if (!kind0 || !kind1 || (ix == 0 && iy == 0) || (three && gap > 1)) {
+    quad_pixel_energy<T, NG>(ak[0], X, Y, o0);
+    if (two) quad_pixel_energy<T, NG>(ak[1], X, Y, o1);
+    return;
+  }
+  const uint64_t col = (uint64_t)iy * a.g.nx + ix;
+  const uint32_t zlo = three ? ((uint32_t)a.band[col] & 0xffffu) : 0;
+  const uint32_t zl = z0[0] < z0[1] ? z0[0] : z0[1];
+  const uint32_t jl = three ? zl - zlo : 0;
+  const bool d[2] = {three && z0[0] > zl, three && z0[1] > zl};
+  const bool wide = d[0] || d[1];  // the bins span NG + 1 band rows
+  const uint64_t slab = (uint64_t)a.J * P2;
+  const cx<T> *own = a.own + (uint64_t)jl * P2 + col, *mir = a.mir + (uint64_t)jl * P2 + col;
+  cx<T> F[2][NG][kEbFields], iso[2][NG];
+#pragma unroll
+  for (int k = 0; k < 2; k++)
+#pragma unroll
+    for (int q = 0; q < NG; q++) {
+      iso[k][q] = mk<T>(0, 0);
+#pragma unroll
+      for (int c = 0; c < kEbFields; c++) F[k][q][c] = mk<T>(0, 0);
+    }
+  // the packed values of volume v at the NG (+ 1) rows, unpacked
+  auto gather = [&](uint32_t v, cx<T> (&xa)[NG + 1], cx<T> (&xb)[NG + 1]) {
+#pragma unroll
+    for (int i = 0; i <= NG; i++) {
+      cx<T> z = mk<T>(0, 0), zm = mk<T>(0, 0);
+      if (i < NG || wide) {
+        z = own[(uint64_t)v * slab + (uint64_t)i * P2];
+        zm = mir[(uint64_t)v * slab + (uint64_t)i * P2];
+      }
+      unpack<T>(z, zm, xa[i], xb[i]);
+    }
+  };
+  auto sel = [&](const cx<T> (&x)[NG + 1], int k, int q) { return d[k] ? x[q + 1] : x[q]; };
+  const uint32_t pairs = a.M / 2;
+  for (uint32_t p = 0; p < pairs; p++) {
+    const MatConst<T> ma[2] = {ak[0].mats[2 * p], ak[1].mats[2 * p]};
+    const MatConst<T> mb[2] = {ak[0].mats[2 * p + 1], ak[1].mats[2 * p + 1]};
+#pragma unroll
+    for (int r = 0; r < 7; r++) {
+      cx<T> xa[NG + 1], xb[NG + 1];
+      gather(7 * p + r, xa, xb);
+#pragma unroll
+      for (int k = 0; k < 2; k++)
+#pragma unroll
+        for (int q = 0; q < NG; q++) {
+          add_basis<T>(r, ma[k], sel(xa, k, q), iso[k][q], F[k][q]);
+          add_basis<T>(r, mb[k], sel(xb, k, q), iso[k][q], F[k][q]);
+        }
+    }
+  }
+  if (a.M & 1) {  // last material alone: (alpha, beta), (q00, q11), (q01, q02), (q12, -)
+    const MatConst<T> mc[2] = {ak[0].mats[a.M - 1], ak[1].mats[a.M - 1]};
+    const uint32_t vb = 7 * pairs;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      cx<T> xa[NG + 1], xb[NG + 1];
+      gather(vb + r, xa, xb);
+#pragma unroll
+      for (int k = 0; k < 2; k++)
+#pragma unroll
+        for (int q = 0; q < NG; q++) {
+          add_basis<T>(2 * r, mc[k], sel(xa, k, q), iso[k][q], F[k][q]);
+          if (r < 3) add_basis<T>(2 * r + 1, mc[k], sel(xb, k, q), iso[k][q], F[k][q]);
+        }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 2; k++) {
+    double md[NG][6];
+#pragma unroll
+    for (int q = 0; q < NG; q++) {
+      F[k][q][0].x += iso[k][q].x;
+      F[k][q][0].y += iso[k][q].y;
+      F[k][q][3].x += iso[k][q].x;
+      F[k][q][3].y += iso[k][q].y;
+      scatter_matrix(ak[k].g, X, Y, three ? z0[k] + q : 0, md[q]);
+    }
+    quad_form_write<T, NG>(a.Q, F[k], md, (T)f[k], k ? o1 : o0);
   }
 }
 
@@ -506,11 +632,25 @@ template <typename T> cyr_status launch_mbasis_angles(const MbAnglesArgs<T> &a, 
   const dim3 grid(ceil_div(P, 256)), block(256);
   if (a.quad) {
     if (!a.kmag || a.ne == 0 || a.ne > 65535) return fail(CYR_E_INVALID, "quadratic form: energy table missing");
-    const uint64_t tiles = (uint64_t)ceil_div(P, 256) * a.ne;
+    // FP32: energies in pairs (one thread per pixel and pair); FP64 (whose pair
+    // kernel would hold 256 registers) and single energies: one at a time
+    const bool paired = a.ne >= 2 && sizeof(T) == 4;
+    const uint32_t per = paired ? (a.ne + 1) / 2 : a.ne;
+    const uint64_t tiles = (uint64_t)ceil_div(P, 256) * per;
     if (tiles > 0x7ffffff0ull) return fail(CYR_E_UNSUPPORTED, "quadratic form: grid too large");
     const dim3 g2(xcd_grid(tiles));
-    if (ng == 2) hipLaunchKernelGGL((mbasis_quad_kernel<T, 2>), g2, block, 0, st, a);
-    else hipLaunchKernelGGL((mbasis_quad_kernel<T, 1>), g2, block, 0, st, a);
+    bool launched = false;
+    if constexpr (sizeof(T) == 4) {
+      if (paired) {
+        if (ng == 2) hipLaunchKernelGGL((mbasis_quad2_kernel<T, 2>), g2, block, 0, st, a);
+        else hipLaunchKernelGGL((mbasis_quad2_kernel<T, 1>), g2, block, 0, st, a);
+        launched = true;
+      }
+    }
+    if (!launched) {
+      if (ng == 2) hipLaunchKernelGGL((mbasis_quad_kernel<T, 2>), g2, block, 0, st, a);
+      else hipLaunchKernelGGL((mbasis_quad_kernel<T, 1>), g2, block, 0, st, a);
+    }
   } else {
     if (ng == 2) hipLaunchKernelGGL((mbasis_angles_kernel<T, 2>), grid, block, 0, st, a);
     else hipLaunchKernelGGL((mbasis_angles_kernel<T, 1>), grid, block, 0, st, a);
