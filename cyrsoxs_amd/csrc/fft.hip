@@ -45,6 +45,7 @@ struct PassArgs {
   double2 *dcz;        // non-null: the Y pass adds the DC-neighbour partial sums (FftDc)
   const void *dc_twz;  // Z twiddles of those sums (null in 2D)
   uint32_t dc_nz;
+  const uint32_t *span;  // Y pass: per output row, the shifted x range it writes (null: all)
 };
 
 // ---- one Stockham stage ------------------------------------------------------
@@ -247,13 +248,39 @@ __global__ void __launch_bounds__(WG) fft_ct_kernel(PassArgs A, const cx<T> *__r
     // are read from the global table, L2 resident)
     cx<T> *ltw = lds + (size_t)A.pp * N;
     for (uint32_t i = threadIdx.x; i < (uint32_t)(LO ? 2 * N : N); i += WG) ltw[i] = tw[i];
+    if (A.span) {  // row spans beside the twiddles
+      uint32_t *ls = reinterpret_cast<uint32_t *>(ltw + (LO ? 2 * N : N));
+      for (uint32_t i = threadIdx.x; i < (uint32_t)N; i += WG) ls[i] = A.span[i];
+    }
     lds_sync();
     twp = ltw;
+  }
+  // Row spans: output row y is written only where some Ewald band reads it, in
+  // whole lines of the block's pp columns (a line is kept when any of its
+  // columns' shifted x coordinates, m - x for bins x <= m = n/2 and n + m - x
+  // above, lies in the row's lo..hi).  Skipping single columns leaves partial
+  // lines, which made the pass slower than writing everything.
+  uint32_t keep = (1u << E) - 1;
+  if (!CONTIG && A.span) {
+    const uint32_t *ls = reinterpret_cast<const uint32_t *>(lds + (size_t)A.pp * N + (LO ? 2 * N : N));
+    const uint32_t n = A.inner_count, m = n / 2;
+    const uint32_t x0 = g * A.pp, x1 = (x0 + A.pp < n ? x0 + A.pp : n) - 1;
+    const bool lo_part = x0 <= m, hi_part = x1 > m;  // the line's bins <= m and > m
+    const uint32_t a0 = m - (x1 < m ? x1 : m), a1 = m - x0;
+    const uint32_t b0 = n + m - x1, b1 = n + m - (x0 > m + 1 ? x0 : m + 1);
+    keep = 0;
+#pragma unroll
+    for (int i = 0; i < E; i++) {
+      const uint32_t sp = ls[F::out_pos(t, i)], lo = sp & 0xffffu, hi = sp >> 16;
+      const bool meet = lo <= hi && ((lo_part && a0 <= hi && a1 >= lo) || (hi_part && b0 <= hi && b1 >= lo));
+      keep |= meet ? 1u << i : 0u;
+    }
   }
   F::run(v, t, p, lds, twp, LO ? twp + N : tw + N);
   if (active) {
 #pragma unroll
-    for (int i = 0; i < E; i++) gbase[(uint64_t)F::out_pos(t, i) * A.elem_stride] = v[i];
+    for (int i = 0; i < E; i++)
+      if (keep & (1u << i)) gbase[(uint64_t)F::out_pos(t, i) * A.elem_stride] = v[i];
   }
 }
 
@@ -281,7 +308,7 @@ hipError_t launch_ct_n(bool contig, uint32_t nc, PassArgs &A, hipStream_t stream
     const uint64_t tiles = (uint64_t)A.groups * A.outer_count * nc;
     if (tiles > 0x7ffffff0ull) return hipErrorInvalidConfiguration;
     constexpr bool LO = tw_lo_in_lds<T>((size_t)PP * N, N, WG);
-    const size_t lds = ((size_t)PP * N + (LO ? 2 * N : N)) * sizeof(cx<T>);
+    const size_t lds = ((size_t)PP * N + (LO ? 2 * N : N)) * sizeof(cx<T>) + (A.span ? 4 * (size_t)N : 0);
     hipLaunchKernelGGL((fft_ct_kernel<T, F, false, WG>), dim3(xcd_grid(tiles)), dim3(WG), lds, stream, A, tw);
   }
   return hipGetLastError();
@@ -529,6 +556,16 @@ template <typename T> bool strided_ct(uint32_t n) {
   return ((size_t)pp * n + n) * sizeof(cx<T>) <= max_lds_bytes();
 }
 
+// Whether the strided codelet's block of length n also holds the row-span table
+// (4 bytes per row) without losing occupancy.
+template <typename T> bool fft_span_fits(uint32_t n) {
+  if (!strided_ct<T>(n)) return false;
+  const uint32_t pp = n <= 512 ? 16 : (n == 1024 ? 8 : 4), wg = pp * (n / ct_elements(n));
+  const bool lo = tw_lo_in_lds<T>((size_t)pp * n, n, wg);
+  const size_t lds = ((size_t)pp * n + (lo ? 2 * n : n)) * sizeof(cx<T>), with = lds + 4 * (size_t)n;
+  return with <= max_lds_bytes() && cu_blocks(with, wg) >= cu_blocks(lds, wg);
+}
+
 // (the strided codelet's block: pp pencils x n / E threads, in waves of 64)
 template <typename T> uint32_t fft_dc_slots(uint32_t nx, uint32_t ny) {
   if (nx < 3 || !strided_ct<T>(ny)) return 0;
@@ -593,6 +630,7 @@ cyr_status fft_axis(void *const *vols, uint32_t count, uint32_t nx, uint32_t ny,
     A.dcz = dc ? dc->dcz + c0 * dc_per_vol : nullptr;
     A.dc_twz = dc ? dc->twz : nullptr;
     A.dc_nz = nz;
+    A.span = dc && dc->span && fft_span_fits<T>(ny) ? dc->span : nullptr;
     bool done = false;
     if (use_ct) {
       bool found = false;
@@ -678,6 +716,8 @@ template cyr_status fft_axis<double>(void *const *, uint32_t, uint32_t, uint32_t
                                      const FftDc *);
 template uint32_t fft_dc_slots<float>(uint32_t, uint32_t);
 template uint32_t fft_dc_slots<double>(uint32_t, uint32_t);
+template bool fft_span_fits<float>(uint32_t);
+template bool fft_span_fits<double>(uint32_t);
 template cyr_status fft3d<float>(void *const *, uint32_t, uint32_t, uint32_t, uint32_t, hipStream_t);
 template cyr_status fft3d<double>(void *const *, uint32_t, uint32_t, uint32_t, uint32_t, hipStream_t);
 

@@ -13,14 +13,21 @@ namespace cyr {
 // fft_axis): dcz receives count x nz x fft_dc_slots(nx, ny) FP64 partials, in
 // [volume][z][slot] order, whose fixed-order sum over one volume is
 //   sum_{y,z} G(1,y,z) + G(Nx-1,y,z) + G(0,y,z) (2 cos(2 pi y/Ny) + 2 cos(2 pi z/Nz)),
-// the z term only in 3D (twz: the Z twiddle table, null in 2D).
+// the z term only in 3D (twz: the Z twiddle table, null in 2D).  With span (Ny
+// entries, lo | (hi << 16)), when fft_span_fits(Ny), the pass writes output row y
+// only in the lines of adjacent columns (16 for Ny <= 512, 8 for 1024, 4 above)
+// holding a column whose shifted x coordinate (the bin -> shifted map m - x,
+// Nx + m - x, m = Nx/2) lies in lo..hi; the other columns keep their input.
 struct FftDc {
   double2 *dcz;
   const void *twz;
+  const uint32_t *span = nullptr;
 };
 // Partials per (volume, z) of a Y pass that can take the DC sums; 0 when it
 // cannot (Nx < 3, or no compile-time plan for Ny).
 template <typename T> uint32_t fft_dc_slots(uint32_t nx, uint32_t ny);
+// Whether a Y pass of length ny honours FftDc::span.
+template <typename T> bool fft_span_fits(uint32_t ny);
 
 // One axis pass over `count` volumes laid out X-fastest with dims (nx, ny, nz).
 // axis 0 = X (contiguous pencils), 1 = Y, 2 = Z (strided pencils).  With dc

@@ -143,10 +143,16 @@ template <typename T> cyr_status launch_ebasis_angles(const AnglesArgs<T> &a, in
 inline uint32_t mbasis_volumes(uint32_t M) { return 7 * (M / 2) + 4 * (M % 2); }
 // Per column (FFT-index order): Zlo | (width << 16) of the shifted z indices the
 // energies [0, ne) sample (kmag[e] = |k|); bmax[0] = widest band, bmax[1..2] = sum
-// of the columns' widths (uint64), then per-block scratch (mbasis_bmax_words in all).
+// of the columns' widths (uint64), bmax[3..4] = columns inside the row spans
+// (uint64), then per-block scratch and the ny row spans (mbasis_bmax_words in all;
+// mbasis_span_table points at the spans).  Row span: lo | (hi << 16), the shifted
+// x coordinates that cover every column whose own or mirrored band is non-empty
+// (lo > hi: none); the Y pass writes and the Z band pass reads only those.
+constexpr uint32_t kBandHead = 5;
 cyr_status launch_mbasis_bands(const EwaldGeom &g, const double *kmag, uint32_t ne, int32_t *band, uint32_t *bmax,
                                hipStream_t st);
 uint64_t mbasis_bmax_words(uint32_t nx, uint32_t ny);
+uint32_t *mbasis_span_table(uint32_t *bmax, uint32_t nx, uint32_t ny);
 // Fields + X FFT of one material pass: a pair (m0, m0 + 1, 7 outputs) or a single
 // material (m0, 4 outputs).
 template <typename T>
@@ -156,6 +162,7 @@ template <typename T> struct BandArgs {
   const cx<T> *vol[kXMaxOut];      // transformed packed volumes of this pass
   uint32_t count, vbase;           // volumes, global index of vol[0]
   const int32_t *band;             // [Ny][Nx] band table
+  const uint32_t *span;            // [Ny] row spans (null: every column)
   cx<T> *own, *mir;                // [volumes][J][Ny][Nx]
   uint32_t J, nx, ny, nz;
 };

@@ -38,7 +38,8 @@ template <> __device__ inline double mb_qnan<double>() { return __longlong_as_do
 // band[col] = Zlo | (width << 16) over every energy's Ewald z index (and its
 // linear-interpolation partner z + 1); 0 = no energy samples the column.  Each
 // block also writes the widest band and the sum of the widths it saw (no atomics:
-// a second kernel reduces): bmax[3 + 2b] = widest, bmax[4 + 2b] = sum of block b.
+// a second kernel reduces): bmax[kBandHead + 2b] = widest, bmax[kBandHead + 2b + 1]
+// = sum of block b.
 __global__ void __launch_bounds__(256) mbasis_bands_kernel(EwaldGeom g, const double *__restrict__ kmag, uint32_t ne,
                                                            int32_t *__restrict__ band, uint32_t *__restrict__ bmax) {
   __shared__ uint32_t red[2][256];
@@ -74,37 +75,97 @@ __global__ void __launch_bounds__(256) mbasis_bands_kernel(EwaldGeom g, const do
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    bmax[3 + 2 * blockIdx.x] = red[0][0];
-    bmax[4 + 2 * blockIdx.x] = red[1][0];
+    bmax[kBandHead + 2 * blockIdx.x] = red[0][0];
+    bmax[kBandHead + 2 * blockIdx.x + 1] = red[1][0];
   }
 }
 
+// ---- columns whose transform some band reads, per row ----------------------------
+// Column c = (x, y) (FFT bins) is read by the band passes when its own band or
+// that of its mirror column (-x, -y) is non-empty.  span[y] = lo | (hi << 16)
+// covers every such column of FFT row y by its shifted x coordinate (unshift is
+// an involution: it also maps bins to shifted coordinates); lo > hi when the row
+// has none.  The Y pass writes, and the Z band pass reads, only columns inside
+// their row's span.  One block per row.
+__global__ void __launch_bounds__(256) mbasis_span_kernel(const int32_t *__restrict__ band, uint32_t nx, uint32_t ny,
+                                                          uint32_t *__restrict__ span) {
+  __shared__ uint32_t red[2][256];
+  const uint32_t y = blockIdx.x, my = y ? ny - y : 0;
+  uint32_t lo = 0xffffu, hi = 0;
+  for (uint32_t x = threadIdx.x; x < nx; x += blockDim.x) {
+    const uint32_t mx = x ? nx - x : 0;
+    if (band[(uint64_t)y * nx + x] != 0 || band[(uint64_t)my * nx + mx] != 0) {
+      const uint32_t X = unshift(x, nx);
+      lo = X < lo ? X : lo;
+      hi = X > hi ? X : hi;
+    }
+  }
+  red[0][threadIdx.x] = lo;
+  red[1][threadIdx.x] = hi;
+  __syncthreads();
+  for (uint32_t s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      red[0][threadIdx.x] = red[0][threadIdx.x + s] < red[0][threadIdx.x] ? red[0][threadIdx.x + s] : red[0][threadIdx.x];
+      red[1][threadIdx.x] = red[1][threadIdx.x + s] > red[1][threadIdx.x] ? red[1][threadIdx.x + s] : red[1][threadIdx.x];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) span[y] = red[0][0] | (red[1][0] << 16);
+}
+
 // bmax[0] = widest band, bmax[1..2] = sum of all widths (uint64, little endian)
-// over the n per-block entries.
-__global__ void __launch_bounds__(256) mbasis_bmax_kernel(uint32_t *__restrict__ bmax, uint32_t n) {
+// over the n per-block entries, bmax[3..4] = columns inside the ny row spans.
+__global__ void __launch_bounds__(256) mbasis_bmax_kernel(uint32_t *__restrict__ bmax, uint32_t n,
+                                                          const uint32_t *__restrict__ span, uint32_t ny) {
   __shared__ uint32_t red[256];
-  __shared__ unsigned long long sum[256];
+  __shared__ unsigned long long sum[2][256];
   uint32_t m = 0;
-  unsigned long long s = 0;
+  unsigned long long s = 0, c = 0;
   for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
-    m = bmax[3 + 2 * i] > m ? bmax[3 + 2 * i] : m;
-    s += bmax[4 + 2 * i];
+    m = bmax[kBandHead + 2 * i] > m ? bmax[kBandHead + 2 * i] : m;
+    s += bmax[kBandHead + 2 * i + 1];
+  }
+  for (uint32_t y = threadIdx.x; y < ny; y += blockDim.x) {
+    const uint32_t lo = span[y] & 0xffffu, hi = span[y] >> 16;
+    if (lo <= hi) c += hi - lo + 1;
   }
   red[threadIdx.x] = m;
-  sum[threadIdx.x] = s;
+  sum[0][threadIdx.x] = s;
+  sum[1][threadIdx.x] = c;
   __syncthreads();
   for (uint32_t k = blockDim.x / 2; k > 0; k >>= 1) {
     if (threadIdx.x < k) {
       red[threadIdx.x] = red[threadIdx.x + k] > red[threadIdx.x] ? red[threadIdx.x + k] : red[threadIdx.x];
-      sum[threadIdx.x] += sum[threadIdx.x + k];
+      sum[0][threadIdx.x] += sum[0][threadIdx.x + k];
+      sum[1][threadIdx.x] += sum[1][threadIdx.x + k];
     }
     __syncthreads();
   }
   if (threadIdx.x == 0) {
     bmax[0] = red[0];
-    bmax[1] = (uint32_t)(sum[0] & 0xffffffffull);
-    bmax[2] = (uint32_t)(sum[0] >> 32);
+    bmax[1] = (uint32_t)(sum[0][0] & 0xffffffffull);
+    bmax[2] = (uint32_t)(sum[0][0] >> 32);
+    bmax[3] = (uint32_t)(sum[1][0] & 0xffffffffull);
+    bmax[4] = (uint32_t)(sum[1][0] >> 32);
   }
+}
+
+// Whether bins x0..x1 (inclusive) of a row meet shifted coordinates lo..hi.
+// The shifted coordinate m - x (x <= m = n/2), n + m - x (x > m) falls on each
+// of the two parts, so each part is a range.
+__device__ inline bool bins_meet_span(uint32_t x0, uint32_t x1, uint32_t n, uint32_t lo, uint32_t hi) {
+  const uint32_t m = n / 2;
+  if (lo > hi) return false;
+  if (x0 <= m) {
+    const uint32_t a = m - (x1 < m ? x1 : m), b = m - x0;
+    if (a <= hi && b >= lo) return true;
+  }
+  if (x1 > m) {
+    const uint32_t s = x0 > m + 1 ? x0 : m + 1;
+    const uint32_t a = n + m - x1, b = n + m - s;
+    if (a <= hi && b >= lo) return true;
+  }
+  return false;
 }
 
 // ---- band values of the transformed packed volumes ----------------------------
@@ -144,16 +205,24 @@ __global__ void __launch_bounds__(WG) mbasis_zband_kernel(BandArgs<T> a, const c
   const uint32_t p = threadIdx.x % PP, t = threadIdx.x / PP;
   const uint32_t v = q / per_vol, tile = q % per_vol;
   const uint32_t g = tile % groups, y = tile / groups;
+  // columns outside the row's span are neither written by the Y pass nor read
+  // here (no band reads them): a tile with none inside leaves (block-uniform,
+  // before any barrier), the others load only their columns inside it
+  const uint32_t sp = a.span ? a.span[y] : 0xffff0000u;
+  const uint32_t slo = sp & 0xffffu, shi = sp >> 16;
+  if (!bins_meet_span(g * PP, (g * PP + PP < a.nx ? g * PP + PP : a.nx) - 1, a.nx, slo, shi)) return;
   const uint32_t x = g * PP + p;
   const bool active = x < a.nx;
   const uint64_t P = (uint64_t)a.nx * a.ny;
   const uint64_t col = (uint64_t)y * a.nx + (active ? x : 0);
+  const uint32_t sx = unshift(active ? x : 0, a.nx);
+  const bool need = active && sx >= slo && sx <= shi;
   const cx<T> *base = a.vol[v] + col;
   // pencil and band-table loads first: they stay in flight across the twiddle
   // staging barrier and the FFT
   cx<T> r[E];
 #pragma unroll
-  for (int i = 0; i < E; i++) r[i] = base[(uint64_t)F::in_pos(t, i) * P];
+  for (int i = 0; i < E; i++) r[i] = need ? base[(uint64_t)F::in_pos(t, i) * P] : mk<T>(0, 0);
   const uint32_t xx = active ? x : 0;
   const uint32_t mx = xx ? a.nx - xx : 0, my = y ? a.ny - y : 0;
   const uint64_t cm = (uint64_t)my * a.nx + mx;
@@ -573,14 +642,21 @@ cyr_status launch_mbasis_bands(const EwaldGeom &g, const double *kmag, uint32_t 
                                hipStream_t st) {
   const uint64_t P = (uint64_t)g.nx * g.ny;
   const uint32_t blocks = ceil_div(P, 256);
+  if (g.nx > 0xfffe) return fail(CYR_E_UNSUPPORTED, "material basis: Nx above 65534");
+  uint32_t *span = mbasis_span_table(bmax, g.nx, g.ny);
   hipLaunchKernelGGL(mbasis_bands_kernel, dim3(blocks), dim3(256), 0, st, g, kmag, ne, band, bmax);
   CYR_LAUNCHED();
-  hipLaunchKernelGGL(mbasis_bmax_kernel, dim3(1), dim3(256), 0, st, bmax, blocks);
+  hipLaunchKernelGGL(mbasis_span_kernel, dim3(g.ny), dim3(256), 0, st, band, g.nx, g.ny, span);
+  CYR_LAUNCHED();
+  hipLaunchKernelGGL(mbasis_bmax_kernel, dim3(1), dim3(256), 0, st, bmax, blocks, span, g.ny);
   CYR_LAUNCHED();
   return CYR_OK;
 }
 
-uint64_t mbasis_bmax_words(uint32_t nx, uint32_t ny) { return 3 + 2 * ceil_div((uint64_t)nx * ny, 256); }
+uint64_t mbasis_bmax_words(uint32_t nx, uint32_t ny) { return kBandHead + 2 * ceil_div((uint64_t)nx * ny, 256) + ny; }
+uint32_t *mbasis_span_table(uint32_t *bmax, uint32_t nx, uint32_t ny) {
+  return bmax + kBandHead + 2 * ceil_div((uint64_t)nx * ny, 256);
+}
 
 template <typename T> cyr_status launch_mbasis_extract(const BandArgs<T> &a, hipStream_t st) {
   if (a.count == 0) return CYR_OK;

@@ -470,7 +470,12 @@ cyr_status run_ebasis(const cyr_sim_desc *d, const SimGeometry &g, const void *d
 // [e0, e0 + ne) for k, by (grid, k, |k| list): known after one eager run, so graph
 // capture can size its launches.
 std::mutex g_band_mu;
-std::map<std::vector<double>, std::pair<uint32_t, uint64_t>> g_band_widths;
+// per band geometry: widest band, sum of the band widths, columns inside the row spans
+struct BandWidths {
+  uint32_t J;
+  uint64_t wsum, ssum;
+};
+std::map<std::vector<double>, BandWidths> g_band_widths;
 
 // Band table of energies [e0, e0 + ne) for k vector k into `band`, its widest band
 // in *J and the sum of the widths in *wsum (read back once per distinct geometry,
@@ -478,7 +483,7 @@ std::map<std::vector<double>, std::pair<uint32_t, uint64_t>> g_band_widths;
 // an earlier run).
 cyr_status mbasis_band(const cyr_sim_desc *d, const SimGeometry &g, uint32_t k, uint32_t e0, uint32_t ne,
                        const double *d_kmag, int32_t *band, uint32_t *bmax, hipStream_t s, uint32_t *J,
-                       uint64_t *wsum) {
+                       uint64_t *wsum, uint64_t *ssum) {
   const double kv[3] = {g.kvec[k].x, g.kvec[k].y, g.kvec[k].z};
   const EwaldGeom geo = make_ewald_geom(g.nx, g.ny, g.nz, d->phys_size, g.kmag[e0], kv, d->interpolation);
   CYR_TRY(launch_mbasis_bands(geo, d_kmag + e0, ne, band, bmax, s));
@@ -489,8 +494,9 @@ cyr_status mbasis_band(const cyr_sim_desc *d, const SimGeometry &g, uint32_t k, 
     std::lock_guard<std::mutex> lk(g_band_mu);
     auto it = g_band_widths.find(key);
     if (it != g_band_widths.end()) {
-      *J = it->second.first;
-      *wsum = it->second.second;
+      *J = it->second.J;
+      *wsum = it->second.wsum;
+      *ssum = it->second.ssum;
       return CYR_OK;
     }
   }
@@ -498,15 +504,16 @@ cyr_status mbasis_band(const cyr_sim_desc *d, const SimGeometry &g, uint32_t k, 
   CYR_HIP(hipStreamIsCapturing(s, &cap));
   if (cap != hipStreamCaptureStatusNone)
     return fail(CYR_E_INVALID, "material basis: Ewald band widths unknown under capture (run the simulation once first)");
-  uint32_t h[3] = {0, 0, 0};
+  uint32_t h[kBandHead] = {};
   CYR_HIP(hipMemcpyAsync(h, bmax, sizeof(h), hipMemcpyDeviceToHost, s));
   CYR_HIP(hipStreamSynchronize(s));
-  const uint64_t sum = (uint64_t)h[1] | ((uint64_t)h[2] << 32);
+  const BandWidths bw{h[0], (uint64_t)h[1] | ((uint64_t)h[2] << 32), (uint64_t)h[3] | ((uint64_t)h[4] << 32)};
   std::lock_guard<std::mutex> lk(g_band_mu);
   if (g_band_widths.size() >= 4096) g_band_widths.clear();  // bounded: entries are cheap to recompute
-  g_band_widths[key] = {h[0], sum};
-  *J = h[0];
-  *wsum = sum;
+  g_band_widths[key] = bw;
+  *J = bw.J;
+  *wsum = bw.wsum;
+  *ssum = bw.ssum;
   return CYR_OK;
 }
 
@@ -573,6 +580,7 @@ cyr_status run_mbasis(const cyr_sim_desc *d, const SimGeometry &g, const void *d
   cx<T> *pol = reinterpret_cast<cx<T> *>(lb.pol.ptr);
   int32_t *band = reinterpret_cast<int32_t *>(lb.bins.ptr);
   uint32_t *bmax = reinterpret_cast<uint32_t *>(band + P);
+  const uint32_t *span = mbasis_span_table(bmax, g.nx, g.ny);
   double2 *dcz = reinterpret_cast<double2 *>(lb.dc.ptr);
   double2 *dcv = dcz + (uint64_t)nvol * dc_per;
   T *proj = reinterpret_cast<T *>(lb.proj.ptr);
@@ -586,9 +594,10 @@ cyr_status run_mbasis(const cyr_sim_desc *d, const SimGeometry &g, const void *d
     for (uint32_t e0 = 0; e0 < g.nE;) {
       uint32_t ne = g.nE - e0, J = 0;
       uint64_t wsum = 0;  // sum over columns of the band widths
+      uint64_t ssum = 0;  // columns inside the row spans (those some band reads)
       for (;;) {
         kt_begin(s);
-        CYR_TRY(mbasis_band(d, g, k, e0, ne, su.kmag, band, bmax, s, &J, &wsum));
+        CYR_TRY(mbasis_band(d, g, k, e0, ne, su.kmag, band, bmax, s, &J, &wsum, &ssum));
         kt_end("mbasis_bands", s, (double)P * 4);
         if (J <= cap || ne == 1) break;
         ne = (ne + 1) / 2;
@@ -622,7 +631,7 @@ cyr_status run_mbasis(const cyr_sim_desc *d, const SimGeometry &g, const void *d
         kt_begin(s);
         CYR_TRY(launch_mbasis_xfields<T>(xa, pair, d_vox, d->storage, d->morphology_type, hann, s));
         kt_end("xfields", s, (pair ? 2.0 : 1.0) * 4.0 * vel * V + nv * (double)V * cxb);
-        FftDc fdc{dcz + (uint64_t)vb * dc_per, su.twz};
+        FftDc fdc{dcz + (uint64_t)vb * dc_per, su.twz, span};
         if (!dc_slots) {
           DcArgs<T> da{};
           for (uint32_t c = 0; c < nv; c++) da.vol[c] = xa.out[c];
@@ -639,12 +648,16 @@ cyr_status run_mbasis(const cyr_sim_desc *d, const SimGeometry &g, const void *d
         }
         kt_begin(s);
         CYR_TRY(fft_axis<T>(vptr, nv, g.nx, g.ny, g.nz, 1, s, dc_slots ? &fdc : nullptr));
-        kt_end("fft_y", s, 2.0 * nv * (double)V * cxb);
+        // reads every volume; writes the columns inside the row spans when it honours
+        // them (algorithmic bytes: the lines at the span edges are written whole)
+        const double ycols = dc_slots && fft_span_fits<T>(g.ny) ? (double)ssum : (double)P;
+        kt_end("fft_y", s, nv * (double)V * cxb + nv * ycols * g.nz * cxb);
         BandArgs<T> ba{};
         for (uint32_t c = 0; c < nv; c++) ba.vol[c] = xa.out[c];
         ba.count = nv;
         ba.vbase = vb;
         ba.band = band;
+        ba.span = span;  // (columns outside it are never read, written by the Y pass or not)
         ba.own = own;
         ba.mir = mir;
         ba.J = J;
@@ -655,9 +668,10 @@ cyr_status run_mbasis(const cyr_sim_desc *d, const SimGeometry &g, const void *d
         if (three) {
           kt_begin(s);
           CYR_TRY(launch_mbasis_zband<T>(ba, s, &fused_z));
-          // reads every volume once; writes each column's band values and those of
-          // its mirror column (own + mir)
-          if (fused_z) kt_end("mbasis_zband", s, nv * (double)V * cxb + 2.0 * nv * (double)wsum * cxb);
+          // reads the volumes' columns inside the row spans; writes each column's band
+          // values and those of its mirror column (own + mir)
+          const double zcols = ba.span ? (double)ssum : (double)P;
+          if (fused_z) kt_end("mbasis_zband", s, nv * zcols * g.nz * cxb + 2.0 * nv * (double)wsum * cxb);
           if (!fused_z) {
             kt_begin(s);
             CYR_TRY(fft_axis<T>(vptr, nv, g.nx, g.ny, g.nz, 2, s));

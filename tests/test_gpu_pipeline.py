@@ -74,6 +74,14 @@ CASES = {
                         angles=(0.0, 0.5, 359.5)),
     "many_angles_rect": dict(dims=(40, 24, 1), M=2, morph=1, kw=dict(interpolation=1, rot_mask=1),
                              angles=(0.0, 0.5, 359.5)),
+    # voxels small enough that the detector's corners lie outside the Ewald sphere
+    # (|q| up to pi / PhysSize above |k| ~ 1.44 nm^-1): the material-basis Y pass
+    # writes, and its Z band pass reads, only the columns some band reads; rows
+    # and whole 32-column tiles with none are skipped
+    "span_partial": dict(dims=(96, 64, 32), M=3, morph=0, kw=dict(interpolation=1, windowing=1), phys=1.2),
+    "span_partial_nn": dict(dims=(64, 80, 16), M=2, morph=1, kw=dict(interpolation=0), phys=1.6),
+    # Ny = 26 has no compile-time plan (no span in the Y pass), Nz = 16 has
+    "span_zband_only": dict(dims=(30, 26, 16), M=2, morph=0, kw=dict(interpolation=1), phys=1.3),
 }
 
 
@@ -89,7 +97,7 @@ def test_modes(case, prec, pipeline):
     mats = random_constants(len(energies), M, 4)
     angles = cfg.get("angles", (0.0, 20.0, 160.0))
     kw = dict(cfg["kw"], morphology_type=cfg["morph"])
-    phys = 1.0 if dims[2] == 1 else 5.0
+    phys = cfg.get("phys", 1.0 if dims[2] == 1 else 5.0)
     osim = O.Sim(dims, M, phys, energies, angles, **kw)
     ref = _oracle(case, osim, vox, mats)
     sim = capi.Sim(dims, M, phys, energies, angles, precision=prec, pipeline=pipeline, **kw)
@@ -155,19 +163,22 @@ def test_host_upload_matches_device_api():
     assert same and err < 2e-3, err
 
 
+@pytest.mark.parametrize("phys", [5.0, 1.4])
 @pytest.mark.parametrize("prec", [capi.FP64, capi.FP32])
-def test_beam_divergence_kvectors(prec):
-    """Case type 1: two tilted k vectors (the reference marks it experimental)."""
+def test_beam_divergence_kvectors(prec, phys):
+    """Case type 1: two tilted k vectors (the reference marks it experimental);
+    at 1.4 nm voxels the Ewald sphere leaves an off-centre part of each detector
+    row unread."""
     dims, M = (24, 24, 24), 2
     vox = random_euler(dims, M, 8)
     mats = random_constants(1, M, 9)
     k = np.array([[0.0, 0.1, 1.0], [0.05, 0.0, 1.0]])
     k = k / np.linalg.norm(k, axis=1, keepdims=True)
     kw = dict(interpolation=1, case_type=1)
-    osim = O.Sim(dims, M, 5.0, [285.0], (0.0, 45.0, 135.0), kvectors=k, **kw)
+    osim = O.Sim(dims, M, phys, [285.0], (0.0, 45.0, 135.0), kvectors=k, **kw)
     ref = O.simulate(osim, vox, mats)
     for pipeline in PIPES:
-        sim = capi.Sim(dims, M, 5.0, [285.0], (0.0, 45.0, 135.0), kvectors=k, precision=prec, pipeline=pipeline,
+        sim = capi.Sim(dims, M, phys, [285.0], (0.0, 45.0, 135.0), kvectors=k, precision=prec, pipeline=pipeline,
                        **kw)
         got = capi.simulate(sim, vox, mats)
         _check(got, ref, prec)
