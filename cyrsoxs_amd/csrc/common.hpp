@@ -182,6 +182,16 @@ inline unsigned xcd_grid(uint64_t tiles) { return ceil_div(tiles, kXcds) * kXcds
 __device__ inline uint32_t xcd_tile(uint32_t w, uint32_t grid) {
   return (w % kXcds) * (grid / kXcds) + w / kXcds;
 }
+// Chunked form for grids whose work varies along the tile range (tiles that
+// leave at once): XCD x walks chunks x, x + 8, x + 16, ... of C consecutive
+// tiles, q = ((i / C) * 8 + x) * C + i mod C with i = w / 8, so neighbours still
+// meet in one L2 and every XCD gets its share of the busy tiles.  The grid is
+// padded to a multiple of 8 C; blocks past the last tile leave.
+inline unsigned xcd_chunk_grid(uint64_t tiles, unsigned C) { return ceil_div(tiles, kXcds * C) * kXcds * C; }
+__device__ inline uint32_t xcd_chunk_tile(uint32_t w, uint32_t C) {
+  const uint32_t i = w / kXcds;
+  return ((i / C) * kXcds + w % kXcds) * C + i % C;
+}
 
 // ---------------------------------------------------------------------------
 // Per-device cached work buffers (grown on demand, freed by

@@ -194,13 +194,17 @@ __global__ void __launch_bounds__(256) mbasis_extract_kernel(BandArgs<T> a) {
 // transformed pencil is not written back: each output Z(col, kz) goes to own[]
 // when kz is in the column's band and to mir[] of the opposite column when -kz
 // is in that column's band.
+constexpr uint32_t kZbandChunk = 16;
 template <typename T, class F, int WG>
 __global__ void __launch_bounds__(WG) mbasis_zband_kernel(BandArgs<T> a, const cx<T> *__restrict__ tw) {
   constexpr int E = F::kE, TPP = F::TPP, N = TPP * E, PP = WG / TPP;
   extern __shared__ unsigned char smem_raw[];
   cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
   const uint32_t groups = (a.nx + PP - 1) / PP, per_vol = groups * a.ny;
-  const uint32_t q = xcd_tile(blockIdx.x, gridDim.x);
+  // chunks of 16 tiles over the XCDs: the tiles outside the row spans leave at
+  // once, and they cluster at the rows far from ky = 0 (one contiguous tile range
+  // per XCD left some XCDs with little work: 426 against 396 us at 512^2 x 128)
+  const uint32_t q = xcd_chunk_tile(blockIdx.x, kZbandChunk);
   if (q >= per_vol * a.count) return;  // grid padding: the whole block leaves before any barrier
   const uint32_t p = threadIdx.x % PP, t = threadIdx.x / PP;
   const uint32_t v = q / per_vol, tile = q % per_vol;
@@ -265,7 +269,8 @@ hipError_t launch_zband_n(const BandArgs<T> &a, hipStream_t st, const cx<T> *tw)
   if (tiles > 0x7ffffff0ull) return hipErrorInvalidConfiguration;
   constexpr bool LO = tw_lo_in_lds<T>((size_t)PP * N, N, WG);
   const size_t lds = ((size_t)PP * N + (LO ? 2 * N : N)) * sizeof(cx<T>);
-  hipLaunchKernelGGL((mbasis_zband_kernel<T, F, WG>), dim3(xcd_grid(tiles)), dim3(WG), lds, st, a, tw);
+  hipLaunchKernelGGL((mbasis_zband_kernel<T, F, WG>), dim3(xcd_chunk_grid(tiles, kZbandChunk)), dim3(WG), lds, st,
+                     a, tw);
   return hipGetLastError();
 }
 
