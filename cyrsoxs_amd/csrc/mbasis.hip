@@ -509,13 +509,16 @@ __device__ inline void quad_pixel_energy(const MbAnglesArgs<T> &a, uint32_t X, u
   quad_form_write<T, NG>(a.Q, F, md, w1, o);
 }
 
+constexpr uint32_t kQuadChunk = 16;  // pixel blocks per XCD chunk of the quadratic-form grids
 template <typename T, int NG>
 __global__ void __launch_bounds__(256) mbasis_quad_kernel(MbAnglesArgs<T> a) {
   const uint64_t P2 = (uint64_t)a.g.nx * a.g.ny;
   // 1D grid in XCD order over (pixel block, energy), energy fastest: the ne
   // energies of a pixel block read the same band columns and run on one XCD
-  // back to back, so all but the first find them in its L2
-  const uint32_t q = xcd_tile(blockIdx.x, gridDim.x);
+  // back to back, so all but the first find them in its L2.  Chunks of 16 pixel
+  // blocks go to the XCDs in turn: the pixels outside the Ewald sphere (cheap)
+  // cluster in the rows far from the beam.
+  const uint32_t q = xcd_chunk_tile(blockIdx.x, kQuadChunk * a.ne);
   const uint32_t e = q % a.ne, b = q / a.ne;
   const uint64_t t = (uint64_t)b * blockDim.x + threadIdx.x;
   if (t >= P2) return;
@@ -535,7 +538,7 @@ template <typename T, int NG>
 __global__ void __launch_bounds__(256) mbasis_quad2_kernel(MbAnglesArgs<T> a) {
   const uint64_t P2 = (uint64_t)a.g.nx * a.g.ny;
   const uint32_t npair = (a.ne + 1) / 2;
-  const uint32_t qt = xcd_tile(blockIdx.x, gridDim.x);
+  const uint32_t qt = xcd_chunk_tile(blockIdx.x, kQuadChunk * npair);  // (as mbasis_quad_kernel)
   const uint32_t ep = qt % npair, b = qt / npair;
   const uint64_t t = (uint64_t)b * blockDim.x + threadIdx.x;
   if (t >= P2) return;
@@ -718,8 +721,9 @@ template <typename T> cyr_status launch_mbasis_angles(const MbAnglesArgs<T> &a, 
     const bool paired = a.ne >= 2 && sizeof(T) == 4;
     const uint32_t per = paired ? (a.ne + 1) / 2 : a.ne;
     const uint64_t tiles = (uint64_t)ceil_div(P, 256) * per;
-    if (tiles > 0x7ffffff0ull) return fail(CYR_E_UNSUPPORTED, "quadratic form: grid too large");
-    const dim3 g2(xcd_grid(tiles));
+    const uint64_t padded = (uint64_t)ceil_div(tiles, (uint64_t)kXcds * kQuadChunk * per) * kXcds * kQuadChunk * per;
+    if (padded > 0x7ffffff0ull) return fail(CYR_E_UNSUPPORTED, "quadratic form: grid too large");
+    const dim3 g2((unsigned)padded);
     bool launched = false;
     if constexpr (sizeof(T) == 4) {
       if (paired) {
