@@ -298,6 +298,46 @@ def test_mbasis_nz256(prec):
     print(f"nz256 prec={prec}: rel L2 {err:.3e}")
 
 
+@pytest.mark.parametrize("phys", [5.0, 1.6])
+def test_mbasis_row_spans(phys):
+    """Columns whose detector pixels lie outside every energy's Ewald sphere are
+    neither written by the Y pass nor read by the Z band pass: with 5 nm voxels
+    (|q| <= pi/5 nm^-1 < |k|) every column is kept, with 1.6 nm voxels the corners
+    drop out (about 43% of the columns kept at 48 x 64).  The patterns match the
+    oracle either way."""
+    dims, M = (48, 64, 64), 2  # (Nz = 64: the fused Z band pass)
+    vox = random_euler(dims, M, 81)
+    energies = [282.0, 285.5]
+    mats = random_constants(len(energies), M, 82)
+    kw = dict(interpolation=1, windowing=1)
+    ref = _oracle(f"spans_{phys}", O.Sim(dims, M, phys, energies, (0.0, 20.0, 160.0), **kw), vox, mats)
+    sim = capi.Sim(dims, M, phys, energies, (0.0, 20.0, 160.0), precision=capi.FP32, pipeline=capi.PIPE_MBASIS,
+                   **kw)
+    capi.kernel_timing(True)
+    try:
+        got = capi.simulate(sim, vox, mats)
+        rep = capi.kernel_timing_report()
+    finally:
+        capi.kernel_timing(False)
+    _check(got, ref, capi.FP32)
+    reads = 7.0 * np.prod(dims) * 8  # one material pair: 7 packed complex64 volumes
+    kept = rep["fft_y"]["avg_bytes"] / reads - 1.0  # written share of the columns
+    print(f"phys {phys}: Y pass keeps {kept:.3f} of the columns; Z band bytes {rep['mbasis_zband']['avg_bytes']:.0f}")
+    # the share of detector pixels inside the Ewald circle of the highest energy
+    # (beam along z: |q_perp| <= |k|, hc = 1239.84 eV nm)
+    kmax = 2 * np.pi * max(energies) / 1239.84193
+    qx = (np.arange(dims[0]) - dims[0] // 2) * 2 * np.pi / (dims[0] * phys)
+    qy = (np.arange(dims[1]) - dims[1] // 2) * 2 * np.pi / (dims[1] * phys)
+    disc = float(np.mean(qx[None, :] ** 2 + qy[:, None] ** 2 <= kmax**2))
+    assert abs(kept - disc) < 0.01, (kept, disc)
+    if phys == 5.0:
+        assert kept > 0.995  # (a detector corner can still fall outside the sampled bins)
+    else:
+        assert 0.3 < kept < 0.6
+        # the Z band pass reads only those columns (plus its band-value writes)
+        assert rep["mbasis_zband"]["avg_bytes"] < 0.7 * reads
+
+
 def test_energy_lanes_bitwise():
     """One or two (energy, k) units in flight give bit-identical patterns."""
     c = synthetic.config0()
