@@ -2,7 +2,9 @@
 API it mirrors (cyr_simulate_device), through the C ABI: the same executors on the
 same inputs give bit-identical patterns.  Two calls in a row share the static
 scratch array, the second with a smaller grid, so stale contents of the first
-call must not leak into the second; a simulation beyond the array's size is
+call must not leak into the second (a third, with voxels small enough that the
+material basis skips detector columns outside the Ewald sphere, follows); a
+simulation beyond the array's size is
 rejected before any device work and leaves the output untouched."""
 import numpy as np
 import pytest
@@ -49,6 +51,16 @@ def test_noalloc_matches_device_api_twice(pipeline, prec):
     assert np.array_equal(got2, ref2, equal_nan=True)
     err, same = pattern_rel_l2(got2, ref2)
     assert same and err == 0.0
+    # call 3: 1.6 nm voxels, so the detector corners fall outside the Ewald sphere
+    # (the material basis's row spans then skip columns; Nz = 64 takes its Z band pass)
+    dims3 = (48, 48, 64)
+    print(f"noalloc pipeline={pipeline} prec={prec}: 48x48x64, 1.6 nm", flush=True)
+    vox3 = random_euler(dims3, 1, 93)
+    mats3 = random_constants(2, 1, 94)
+    sim3 = capi.Sim(dims3, 1, 1.6, [283.0, 286.5], (0.0, 20.0, 160.0), interpolation=1, precision=prec,
+                    pipeline=pipeline)
+    got3, ref3 = _pair(sim3, vox3, mats3, dt)
+    assert np.array_equal(got3, ref3, equal_nan=True)
 
 
 def test_noalloc_rejects_beyond_scratch():
