@@ -78,10 +78,15 @@ CASES = {
     # (|q| up to pi / PhysSize above |k| ~ 1.44 nm^-1): the material-basis Y pass
     # writes, and its Z band pass reads, only the columns some band reads; rows
     # and whole 32-column tiles with none are skipped
-    "span_partial": dict(dims=(96, 64, 32), M=3, morph=0, kw=dict(interpolation=1, windowing=1), phys=1.2),
-    "span_partial_nn": dict(dims=(64, 80, 16), M=2, morph=1, kw=dict(interpolation=0), phys=1.6),
-    # Ny = 26 has no compile-time plan (no span in the Y pass), Nz = 16 has
-    "span_zband_only": dict(dims=(30, 26, 16), M=2, morph=0, kw=dict(interpolation=1), phys=1.3),
+    "span_partial": dict(dims=(96, 64, 64), M=3, morph=0, kw=dict(interpolation=1, windowing=1), phys=1.2),
+    # Nx = 80: the last 32-column tile of each row is ragged
+    "span_partial_nn": dict(dims=(80, 64, 128), M=2, morph=1, kw=dict(interpolation=0), phys=1.6),
+    # Ny = 26 has no compile-time plan (no span in the Y pass), Nz = 64 has
+    "span_zband_only": dict(dims=(30, 26, 64), M=2, morph=0, kw=dict(interpolation=1), phys=1.3),
+    # Nz = 32: no Z band pass (Z FFT pass + band extraction behind the spanned Y pass)
+    "span_extract": dict(dims=(64, 64, 32), M=2, morph=0, kw=dict(interpolation=1), phys=1.4),
+    "span_material_frame": dict(dims=(64, 48, 64), M=2, morph=0, kw=dict(interpolation=1, reference_frame=0),
+                                phys=1.5),
 }
 
 
