@@ -493,6 +493,21 @@ __global__ void __launch_bounds__(256) detector_warp_kernel(const double *__rest
   out[t] = (T)v;
 }
 
+// The same for an identity warp (the detector normal along the beam): every
+// destination pixel samples its own source pixel with weight 1, so the pattern
+// is the average converted (0 + a, as the weighted sum forms it).
+template <typename T>
+__global__ void __launch_bounds__(256) detector_copy_kernel(const double *__restrict__ avg, T *__restrict__ out,
+                                                            uint64_t P, uint64_t out_stride) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= P) return;
+  out[(uint64_t)blockIdx.y * out_stride + t] = (T)(0.0 + avg[(uint64_t)blockIdx.y * P + t]);
+}
+
+bool is_identity(const double inv[6]) {
+  return inv[0] == 1.0 && inv[1] == 0.0 && inv[2] == 0.0 && inv[3] == 0.0 && inv[4] == 1.0 && inv[5] == 0.0;
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) warp_kernel(const T *__restrict__ src, T *__restrict__ dst, uint32_t nx,
                                                    uint32_t ny, Inv6 inv) {
@@ -641,8 +656,11 @@ template <typename T>
 cyr_status launch_detector_warp(const double *avg, T *out, uint32_t nx, uint32_t ny, const double inv[6], int rot_mask,
                                 hipStream_t st) {
   const uint64_t P = (uint64_t)nx * ny;
-  hipLaunchKernelGGL((detector_warp_kernel<T>), dim3(ceil_div(P, 256)), dim3(256), 0, st, avg, out, nx, ny,
-                     mk_inv(inv), rot_mask, (uint64_t)0);
+  if (is_identity(inv))
+    hipLaunchKernelGGL((detector_copy_kernel<T>), dim3(ceil_div(P, 256)), dim3(256), 0, st, avg, out, P, (uint64_t)0);
+  else
+    hipLaunchKernelGGL((detector_warp_kernel<T>), dim3(ceil_div(P, 256)), dim3(256), 0, st, avg, out, nx, ny,
+                       mk_inv(inv), rot_mask, (uint64_t)0);
   CYR_LAUNCHED();
   return CYR_OK;
 }
@@ -652,8 +670,12 @@ cyr_status launch_detector_warp_n(const double *avg, T *out, uint64_t out_stride
                                   uint32_t ny, const double inv[6], int rot_mask, hipStream_t st) {
   const uint64_t P = (uint64_t)nx * ny;
   if (ne == 0) return CYR_OK;
-  hipLaunchKernelGGL((detector_warp_kernel<T>), dim3(ceil_div(P, 256), ne), dim3(256), 0, st, avg, out, nx, ny,
-                     mk_inv(inv), rot_mask, out_stride);
+  if (is_identity(inv))
+    hipLaunchKernelGGL((detector_copy_kernel<T>), dim3(ceil_div(P, 256), ne), dim3(256), 0, st, avg, out, P,
+                       out_stride);
+  else
+    hipLaunchKernelGGL((detector_warp_kernel<T>), dim3(ceil_div(P, 256), ne), dim3(256), 0, st, avg, out, nx, ny,
+                       mk_inv(inv), rot_mask, out_stride);
   CYR_LAUNCHED();
   return CYR_OK;
 }
