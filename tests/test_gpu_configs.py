@@ -121,10 +121,12 @@ def test_served_request_matches_oracle():
             print(f"  rel L2 = {err:.3e}")
 
 
+@pytest.mark.parametrize("phys", [5.0, 1.5])
 @pytest.mark.parametrize("pipeline", [capi.PIPE_PLAIN, capi.PIPE_EBASIS, capi.PIPE_MBASIS])
 @pytest.mark.parametrize("prec", [capi.FP64, capi.FP32])
-def test_grazing_incidence(prec, pipeline):
-    """Case type 2: k tilted off z and the detector normal tilted the other way."""
+def test_grazing_incidence(prec, pipeline, phys):
+    """Case type 2: k tilted off z and the detector normal tilted the other way (at
+    1.5 nm voxels part of the detector lies outside the Ewald sphere)."""
     dims, M = (32, 32, 16), 2
     vox = random_euler(dims, M, 71)
     mats = random_constants(2, M, 72)
@@ -133,16 +135,16 @@ def test_grazing_incidence(prec, pipeline):
     det = np.array([0.15, -0.05, 1.0])
     det /= np.linalg.norm(det)
     kw = dict(interpolation=1, case_type=2)
-    key = ("grazing",)
+    key = ("grazing", phys)
     if key not in _CACHE:
-        osim = O.Sim(dims, M, 5.0, [284.0, 287.0], (0.0, 30.0, 150.0), kvectors=k, detector=det, **kw)
+        osim = O.Sim(dims, M, phys, [284.0, 287.0], (0.0, 30.0, 150.0), kvectors=k, detector=det, **kw)
         _CACHE[key] = O.simulate(osim, vox, mats)
     ref = _CACHE[key]
     # the detector warp moves the pattern: it differs from the untilted detector's
-    flat = O.simulate(O.Sim(dims, M, 5.0, [284.0, 287.0], (0.0, 30.0, 150.0), kvectors=k, **kw), vox, mats)
+    flat = O.simulate(O.Sim(dims, M, phys, [284.0, 287.0], (0.0, 30.0, 150.0), kvectors=k, **kw), vox, mats)
     assert not np.array_equal(np.isnan(flat), np.isnan(ref))
-    print(f"grazing prec={prec} pipeline={pipeline}", flush=True)
-    sim = capi.Sim(dims, M, 5.0, [284.0, 287.0], (0.0, 30.0, 150.0), kvectors=k, detector=det, precision=prec,
+    print(f"grazing prec={prec} pipeline={pipeline} phys={phys}", flush=True)
+    sim = capi.Sim(dims, M, phys, [284.0, 287.0], (0.0, 30.0, 150.0), kvectors=k, detector=det, precision=prec,
                    pipeline=pipeline, **kw)
     err = _check(capi.simulate(sim, vox, mats), ref, prec)
     print(f"  rel L2 = {err:.3e}")

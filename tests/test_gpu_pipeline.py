@@ -87,6 +87,8 @@ CASES = {
     "span_extract": dict(dims=(64, 64, 32), M=2, morph=0, kw=dict(interpolation=1), phys=1.4),
     "span_material_frame": dict(dims=(64, 48, 64), M=2, morph=0, kw=dict(interpolation=1, reference_frame=0),
                                 phys=1.5),
+    # odd Nx (run-time X plan; the bin -> shifted map of odd lengths; a ragged last tile)
+    "span_odd_nx": dict(dims=(45, 64, 64), M=2, morph=1, kw=dict(interpolation=1, windowing=1), phys=1.3),
 }
 
 
