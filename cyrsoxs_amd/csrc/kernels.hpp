@@ -144,11 +144,14 @@ inline uint32_t mbasis_volumes(uint32_t M) { return 7 * (M / 2) + 4 * (M % 2); }
 // Per column (FFT-index order): Zlo | (width << 16) of the shifted z indices the
 // energies [0, ne) sample (kmag[e] = |k|); bmax[0] = widest band, bmax[1..2] = sum
 // of the columns' widths (uint64), bmax[3..4] = columns inside the row spans
-// (uint64), then per-block scratch and the ny row spans (mbasis_bmax_words in all;
+// (uint64), bmax[5..6] = D4 (uint64): the largest (2X - Nx)^2 + (2Y - Ny)^2 over the
+// span endpoints (X shifted column, Y shifted row), so no detector pixel farther
+// than sqrt(D4) / 2 from the plane's centre has a finite value in any energy of
+// the group; then per-block scratch and the ny row spans (mbasis_bmax_words in all;
 // mbasis_span_table points at the spans).  Row span: lo | (hi << 16), the shifted
 // x coordinates that cover every column whose own or mirrored band is non-empty
 // (lo > hi: none); the Y pass writes and the Z band pass reads only those.
-constexpr uint32_t kBandHead = 5;
+constexpr uint32_t kBandHead = 7;
 cyr_status launch_mbasis_bands(const EwaldGeom &g, const double *kmag, uint32_t ne, int32_t *band, uint32_t *bmax,
                                hipStream_t st);
 uint64_t mbasis_bmax_words(uint32_t nx, uint32_t ny);
@@ -228,6 +231,9 @@ cyr_status launch_rotate_all(const T *proj, const double *inv_tab, uint32_t nA, 
 // Quarter-turn orbits of a square N x N plane (N even): orbit representatives
 // in 16 x 16 tiles (ty << 16 | tx) ordered by distance from the rotation centre.
 uint32_t rotation_orbit_tile_count(uint32_t n);
+// Orbit tiles whose centres lie within r pixels of the plane's centre (a prefix
+// of rotation_orbit_tile_order).
+uint32_t rotation_orbit_tiles_within(uint32_t n, double r);
 void rotation_orbit_tile_order(uint32_t n, uint32_t *out);
 // Quarter turns Q (2..4) that the orbit kernel can share taps over, given
 // `quarter` angle steps per 90 degrees (0: none); 0 when it does not apply.
@@ -239,10 +245,13 @@ uint32_t rotation_quarter_turns(uint32_t nx, uint32_t ny, uint32_t nA, uint32_t 
 // steps per 90 degrees, 0 = none) with otiles (orbit tile order) lets each tap
 // cell serve the angles a quarter turn apart; share: coef_t of angles a and
 // a + 2 quarter turns are equal (then one intensity serves both pairs).
+// rfin >= 0: no plane value farther than rfin pixels from the centre is finite
+// (the orbit path then fills the orbit tiles beyond it without sampling: every
+// tap there is NaN, so their averages are NaN, or 0 with the rotation mask).
 template <typename T>
 cyr_status launch_rotate_quad(const T *quad, const double *inv_tab, const double *coef, const T *coef_t, uint32_t nA,
                               uint32_t nx, uint32_t ny, int rot_mask, double *avg, uint32_t ne, const uint32_t *tiles,
-                              uint32_t quarter, const uint32_t *otiles, bool share, hipStream_t st);
+                              uint32_t quarter, const uint32_t *otiles, bool share, double rfin, hipStream_t st);
 // Detector warp of the averaged pattern into the output (prefill 0 or NaN).
 template <typename T>
 cyr_status launch_detector_warp(const double *avg, T *out, uint32_t nx, uint32_t ny, const double inv[6],

@@ -114,30 +114,39 @@ __global__ void __launch_bounds__(256) mbasis_span_kernel(const int32_t *__restr
 }
 
 // bmax[0] = widest band, bmax[1..2] = sum of all widths (uint64, little endian)
-// over the n per-block entries, bmax[3..4] = columns inside the ny row spans.
+// over the n per-block entries, bmax[3..4] = columns inside the ny row spans,
+// bmax[5..6] = D4, the largest (2X - nx)^2 + (2Y - ny)^2 over the spans' end
+// columns (X: shifted column, Y = unshift(y): the shifted row of FFT row y).
 __global__ void __launch_bounds__(256) mbasis_bmax_kernel(uint32_t *__restrict__ bmax, uint32_t n,
-                                                          const uint32_t *__restrict__ span, uint32_t ny) {
+                                                          const uint32_t *__restrict__ span, uint32_t nx, uint32_t ny) {
   __shared__ uint32_t red[256];
-  __shared__ unsigned long long sum[2][256];
+  __shared__ unsigned long long sum[3][256];
   uint32_t m = 0;
-  unsigned long long s = 0, c = 0;
+  unsigned long long s = 0, c = 0, d4 = 0;
   for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
     m = bmax[kBandHead + 2 * i] > m ? bmax[kBandHead + 2 * i] : m;
     s += bmax[kBandHead + 2 * i + 1];
   }
   for (uint32_t y = threadIdx.x; y < ny; y += blockDim.x) {
     const uint32_t lo = span[y] & 0xffffu, hi = span[y] >> 16;
-    if (lo <= hi) c += hi - lo + 1;
+    if (lo <= hi) {
+      c += hi - lo + 1;
+      const long long dy = 2ll * unshift(y, ny) - ny, dl = 2ll * lo - nx, dh = 2ll * hi - nx;
+      const unsigned long long r = (unsigned long long)(dy * dy + (dl * dl > dh * dh ? dl * dl : dh * dh));
+      d4 = r > d4 ? r : d4;
+    }
   }
   red[threadIdx.x] = m;
   sum[0][threadIdx.x] = s;
   sum[1][threadIdx.x] = c;
+  sum[2][threadIdx.x] = d4;
   __syncthreads();
   for (uint32_t k = blockDim.x / 2; k > 0; k >>= 1) {
     if (threadIdx.x < k) {
       red[threadIdx.x] = red[threadIdx.x + k] > red[threadIdx.x] ? red[threadIdx.x + k] : red[threadIdx.x];
       sum[0][threadIdx.x] += sum[0][threadIdx.x + k];
       sum[1][threadIdx.x] += sum[1][threadIdx.x + k];
+      sum[2][threadIdx.x] = sum[2][threadIdx.x + k] > sum[2][threadIdx.x] ? sum[2][threadIdx.x + k] : sum[2][threadIdx.x];
     }
     __syncthreads();
   }
@@ -147,6 +156,8 @@ __global__ void __launch_bounds__(256) mbasis_bmax_kernel(uint32_t *__restrict__
     bmax[2] = (uint32_t)(sum[0][0] >> 32);
     bmax[3] = (uint32_t)(sum[1][0] & 0xffffffffull);
     bmax[4] = (uint32_t)(sum[1][0] >> 32);
+    bmax[5] = (uint32_t)(sum[2][0] & 0xffffffffull);
+    bmax[6] = (uint32_t)(sum[2][0] >> 32);
   }
 }
 
@@ -656,7 +667,7 @@ cyr_status launch_mbasis_bands(const EwaldGeom &g, const double *kmag, uint32_t 
   CYR_LAUNCHED();
   hipLaunchKernelGGL(mbasis_span_kernel, dim3(g.ny), dim3(256), 0, st, band, g.nx, g.ny, span);
   CYR_LAUNCHED();
-  hipLaunchKernelGGL(mbasis_bmax_kernel, dim3(1), dim3(256), 0, st, bmax, blocks, span, g.ny);
+  hipLaunchKernelGGL(mbasis_bmax_kernel, dim3(1), dim3(256), 0, st, bmax, blocks, span, g.nx, g.ny);
   CYR_LAUNCHED();
   return CYR_OK;
 }

@@ -470,10 +470,11 @@ cyr_status run_ebasis(const cyr_sim_desc *d, const SimGeometry &g, const void *d
 // [e0, e0 + ne) for k, by (grid, k, |k| list): known after one eager run, so graph
 // capture can size its launches.
 std::mutex g_band_mu;
-// per band geometry: widest band, sum of the band widths, columns inside the row spans
+// per band geometry: widest band, sum of the band widths, columns inside the row
+// spans, D4 (no finite plane value beyond sqrt(D4) / 2 pixels of the centre)
 struct BandWidths {
   uint32_t J;
-  uint64_t wsum, ssum;
+  uint64_t wsum, ssum, d4;
 };
 std::map<std::vector<double>, BandWidths> g_band_widths;
 
@@ -483,7 +484,7 @@ std::map<std::vector<double>, BandWidths> g_band_widths;
 // an earlier run).
 cyr_status mbasis_band(const cyr_sim_desc *d, const SimGeometry &g, uint32_t k, uint32_t e0, uint32_t ne,
                        const double *d_kmag, int32_t *band, uint32_t *bmax, hipStream_t s, uint32_t *J,
-                       uint64_t *wsum, uint64_t *ssum) {
+                       uint64_t *wsum, uint64_t *ssum, uint64_t *d4) {
   const double kv[3] = {g.kvec[k].x, g.kvec[k].y, g.kvec[k].z};
   const EwaldGeom geo = make_ewald_geom(g.nx, g.ny, g.nz, d->phys_size, g.kmag[e0], kv, d->interpolation);
   CYR_TRY(launch_mbasis_bands(geo, d_kmag + e0, ne, band, bmax, s));
@@ -497,6 +498,7 @@ cyr_status mbasis_band(const cyr_sim_desc *d, const SimGeometry &g, uint32_t k, 
       *J = it->second.J;
       *wsum = it->second.wsum;
       *ssum = it->second.ssum;
+      *d4 = it->second.d4;
       return CYR_OK;
     }
   }
@@ -507,13 +509,15 @@ cyr_status mbasis_band(const cyr_sim_desc *d, const SimGeometry &g, uint32_t k, 
   uint32_t h[kBandHead] = {};
   CYR_HIP(hipMemcpyAsync(h, bmax, sizeof(h), hipMemcpyDeviceToHost, s));
   CYR_HIP(hipStreamSynchronize(s));
-  const BandWidths bw{h[0], (uint64_t)h[1] | ((uint64_t)h[2] << 32), (uint64_t)h[3] | ((uint64_t)h[4] << 32)};
+  const BandWidths bw{h[0], (uint64_t)h[1] | ((uint64_t)h[2] << 32), (uint64_t)h[3] | ((uint64_t)h[4] << 32),
+                      (uint64_t)h[5] | ((uint64_t)h[6] << 32)};
   std::lock_guard<std::mutex> lk(g_band_mu);
   if (g_band_widths.size() >= 4096) g_band_widths.clear();  // bounded: entries are cheap to recompute
   g_band_widths[key] = bw;
   *J = bw.J;
   *wsum = bw.wsum;
   *ssum = bw.ssum;
+  *d4 = bw.d4;
   return CYR_OK;
 }
 
@@ -595,9 +599,10 @@ cyr_status run_mbasis(const cyr_sim_desc *d, const SimGeometry &g, const void *d
       uint32_t ne = g.nE - e0, J = 0;
       uint64_t wsum = 0;  // sum over columns of the band widths
       uint64_t ssum = 0;  // columns inside the row spans (those some band reads)
+      uint64_t d4 = 0;    // (2 x the largest distance of a finite plane value from the centre)^2
       for (;;) {
         kt_begin(s);
-        CYR_TRY(mbasis_band(d, g, k, e0, ne, su.kmag, band, bmax, s, &J, &wsum, &ssum));
+        CYR_TRY(mbasis_band(d, g, k, e0, ne, su.kmag, band, bmax, s, &J, &wsum, &ssum, &d4));
         kt_end("mbasis_bands", s, (double)P * 4);
         if (J <= cap || ne == 1) break;
         ne = (ne + 1) / 2;
@@ -718,7 +723,7 @@ cyr_status run_mbasis(const cyr_sim_desc *d, const SimGeometry &g, const void *d
           kt_begin(s);
           CYR_TRY(launch_rotate_quad<T>(proj, inv, su.quad + (size_t)3 * k * g.nA, su.quad_t + (size_t)3 * k * g.nA,
                                         g.nA, g.nx, g.ny, d->rot_mask, avg, n, su.tiles, g.quarter, su.otiles,
-                                        quad_pairs_share<T>(g, k), s));
+                                        quad_pairs_share<T>(g, k), 0.5 * std::sqrt((double)d4), s));
           kt_end("rotate_quad", s, n * ((double)kQuadRecord * P * sizeof(T) + 8.0 * P));
           CYR_TRY(launch_detector_warp_n<T>(avg, d_out + ((uint64_t)b0 * g.nK + k) * P, (uint64_t)g.nK * P, n, g.nx,
                                             g.ny, &g.det_inv[(size_t)k * 6], d->rot_mask, s));

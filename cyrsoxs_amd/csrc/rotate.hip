@@ -477,6 +477,32 @@ __global__ void __launch_bounds__(256) rotate_orbit_kernel(const T *__restrict__
   }
 }
 
+// Orbit tiles beyond every finite plane value (their pixels' taps, within 1.5
+// pixels of points as far from the centre as the pixel, are all NaN): the
+// averages rotate_orbit_kernel would form there, NaN, or 0 with the rotation
+// mask (no valid sample).  Same tile and pixel layout as rotate_orbit_kernel.
+template <int NE, bool RM>
+__global__ void __launch_bounds__(256) orbit_fill_kernel(double *__restrict__ avg, uint32_t n, uint32_t ne,
+                                                         const uint32_t *__restrict__ otiles, uint32_t first) {
+  const uint64_t P = (uint64_t)n * n;
+  const uint32_t h = n / 2;
+  const uint32_t e0 = blockIdx.z * NE;
+  const uint32_t tt = otiles[first + blockIdx.x];
+  const uint32_t w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const uint32_t u = (tt & 0xffffu) * 16 + (w & 1) * 8 + (l & 7) + 1;
+  const uint32_t v = (tt >> 16) * 16 + (w >> 1) * 8 + (l >> 3);
+  if (u >= h || v >= h) return;
+  const uint32_t X0 = h + u, Y0 = h + v;
+  const uint32_t PX[4] = {X0, n - Y0, n - X0, Y0};
+  const uint32_t PY[4] = {Y0, X0, n - Y0, n - X0};
+  const double fill = RM ? 0.0 : __longlong_as_double(0x7ff8000000000000ll);
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int e = 0; e < NE; e++)
+      if (e0 + e < ne) avg[(uint64_t)(e0 + e) * P + (uint64_t)PY[i] * n + PX[i]] = fill;
+}
+
 // grid y = pattern: avg + y P -> out + y out_stride
 template <typename T>
 __global__ void __launch_bounds__(256) detector_warp_kernel(const double *__restrict__ avg, T *__restrict__ out,
@@ -558,6 +584,19 @@ void rotation_orbit_tile_order(uint32_t n, uint32_t *out) {
   for (size_t i = 0; i < r.size(); i++) out[i] = r[i].second;
 }
 
+// Orbit tiles whose centre (du, dv of rotation_orbit_tile_order) lies within r
+// pixels of the plane's centre: the first ones of that order.
+uint32_t rotation_orbit_tiles_within(uint32_t n, double r) {
+  const uint32_t h = n / 2, tx = ceil_div(h - 1, 16), ty = ceil_div(h, 16);
+  uint32_t c = 0;
+  for (uint32_t y = 0; y < ty; y++)
+    for (uint32_t x = 0; x < tx; x++) {
+      const double du = x * 16.0 + 9.0, dv = y * 16.0 + 8.0;
+      c += du * du + dv * dv <= r * r ? 1u : 0u;
+    }
+  return c;
+}
+
 uint32_t rotation_quarter_turns(uint32_t nx, uint32_t ny, uint32_t nA, uint32_t quarter) {
   if (quarter == 0 || nx != ny || nx < 4 || nx % 2 != 0 || nA % quarter != 0) return 0;
   const uint32_t q = nA / quarter;
@@ -616,7 +655,7 @@ hipError_t launch_orbit_rm(uint32_t Q, dim3 grid, hipStream_t st, const T *quad,
 template <typename T>
 cyr_status launch_rotate_quad(const T *quad, const double *inv_tab, const double *coef, const T *coef_t, uint32_t nA,
                               uint32_t nx, uint32_t ny, int rot_mask, double *avg, uint32_t ne, const uint32_t *tiles,
-                              uint32_t quarter, const uint32_t *otiles, bool share, hipStream_t st) {
+                              uint32_t quarter, const uint32_t *otiles, bool share, double rfin, hipStream_t st) {
   if (ne == 0) return CYR_OK;
   if ((uint64_t)nx * ny > 0xffffffffull) return fail(CYR_E_UNSUPPORTED, "rotation: more than 2^32 pixels");
   if (!tiles) return fail(CYR_E_INVALID, "rotation: tile order missing");
@@ -626,16 +665,27 @@ cyr_status launch_rotate_quad(const T *quad, const double *inv_tab, const double
   if (Q) {
     // quarter-turn orbits: each gathered tap cell serves Q (pixel, angle) pairs
     const uint32_t n_otiles = rotation_orbit_tile_count(nx);
+    // tiles (in order of distance from the centre) whose pixels all lie more than
+    // rfin + 2 pixels out are filled, not sampled: a tile's pixels are within
+    // 8 sqrt(2) < 11.4 pixels of its centre, hence the cut at rfin + 14
+    const uint32_t n_act = rfin >= 0 ? std::min(n_otiles, rotation_orbit_tiles_within(nx, rfin + 14.0)) : n_otiles;
     // 2 energies per pass (measured against 1 and 4: 512^2 x 20 energies 415 / 384 us
     // for 4 / 2; 2048^2 film 7.30 / 6.75 / 8.92 ms per 10 energies for 1 / 2 / 4)
-    const uint64_t blocks = (uint64_t)n_otiles + ceil_div(2ull * nx, 256);
+    const uint64_t blocks = (uint64_t)n_act + ceil_div(2ull * nx, 256);
     const dim3 grid(xcd_grid(blocks), 1, ceil_div(ne, 2));
-    const hipError_t e =
+    hipError_t e =
         rot_mask ? launch_orbit_rm<T, 2, true>(Q, grid, st, quad, inv_tab, coef, coef_t, nA, quarter, nx, avg, ne,
-                                                otiles, n_otiles, share ? 1 : 0)
+                                                otiles, n_act, share ? 1 : 0)
                  : launch_orbit_rm<T, 2, false>(Q, grid, st, quad, inv_tab, coef, coef_t, nA, quarter, nx, avg, ne,
-                                                 otiles, n_otiles, share ? 1 : 0);
+                                                 otiles, n_act, share ? 1 : 0);
     if (e != hipSuccess) return hip_fail(e, "rotate_orbit_kernel", __FILE__, __LINE__);
+    if (n_act < n_otiles) {
+      const dim3 gf(n_otiles - n_act, 1, ceil_div(ne, 2));
+      if (rot_mask) hipLaunchKernelGGL((orbit_fill_kernel<2, true>), gf, dim3(256), 0, st, avg, nx, ne, otiles, n_act);
+      else hipLaunchKernelGGL((orbit_fill_kernel<2, false>), gf, dim3(256), 0, st, avg, nx, ne, otiles, n_act);
+      e = hipGetLastError();
+      if (e != hipSuccess) return hip_fail(e, "orbit_fill_kernel", __FILE__, __LINE__);
+    }
     return CYR_OK;
   }
   const uint32_t ntiles = rotation_tile_count(nx, ny);
@@ -695,7 +745,7 @@ cyr_status launch_warp(const T *src, T *dst, uint32_t nx, uint32_t ny, const dou
                                            const uint32_t *, hipStream_t);                                         \
   template cyr_status launch_rotate_quad<T>(const T *, const double *, const double *, const T *, uint32_t,      \
                                             uint32_t, uint32_t, int, double *, uint32_t, const uint32_t *,        \
-                                            uint32_t, const uint32_t *, bool, hipStream_t);                        \
+                                            uint32_t, const uint32_t *, bool, double, hipStream_t);                \
   template cyr_status launch_detector_warp_n<T>(const double *, T *, uint64_t, uint32_t, uint32_t, uint32_t,       \
                                                 const double *, int, hipStream_t);                                 \
   template cyr_status launch_detector_warp<T>(const double *, T *, uint32_t, uint32_t, const double *, int,        \
