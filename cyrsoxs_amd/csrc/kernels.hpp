@@ -222,6 +222,9 @@ cyr_status launch_average(const double *acc, const uint32_t *count, double *avg,
 // Output tiles (16 x 16) of the rotation kernels, ordered by distance from the
 // rotation centre (ty << 16 | tx); the kernels give XCD x the x-th annulus.
 uint32_t rotation_tile_count(uint32_t nx, uint32_t ny);
+// Tiles of rotation_tile_order whose centres lie within r pixels of the plane's
+// centre (a prefix of that order).
+uint32_t rotation_tiles_within(uint32_t nx, uint32_t ny, double r);
 void rotation_tile_order(uint32_t nx, uint32_t ny, uint32_t *out);
 // E-basis path: rotate every angle's projection, accumulate and average in one pass
 // (tiles: rotation_tile_order on the device).
@@ -246,8 +249,8 @@ uint32_t rotation_quarter_turns(uint32_t nx, uint32_t ny, uint32_t nA, uint32_t 
 // cell serve the angles a quarter turn apart; share: coef_t of angles a and
 // a + 2 quarter turns are equal (then one intensity serves both pairs).
 // rfin >= 0: no plane value farther than rfin pixels from the centre is finite
-// (the orbit path then fills the orbit tiles beyond it without sampling: every
-// tap there is NaN, so their averages are NaN, or 0 with the rotation mask).
+// (the tiles beyond it are then filled without sampling: every tap there is NaN,
+// so their averages are NaN, or 0 with the rotation mask).
 template <typename T>
 cyr_status launch_rotate_quad(const T *quad, const double *inv_tab, const double *coef, const T *coef_t, uint32_t nA,
                               uint32_t nx, uint32_t ny, int rot_mask, double *avg, uint32_t ne, const uint32_t *tiles,

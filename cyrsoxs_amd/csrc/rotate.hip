@@ -260,6 +260,26 @@ __global__ void __launch_bounds__(256) rotate_quad_kernel(const T *__restrict__ 
   quad_store<NE>(avg, P, (uint64_t)Y * nx + X, e0, ne, nA, rot_mask, acc, cnt);
 }
 
+// Tiles of the general order beyond every finite plane value: the averages
+// rotate_quad_kernel would form there (NaN, or 0 with the rotation mask), as in
+// orbit_fill_kernel.  Same tile and pixel layout as tile_pixel.
+template <int NE>
+__global__ void __launch_bounds__(256) quad_fill_kernel(double *__restrict__ avg, uint32_t nx, uint32_t ny,
+                                                        uint32_t ne, int rot_mask, const uint32_t *__restrict__ tiles,
+                                                        uint32_t first) {
+  const uint64_t P = (uint64_t)nx * ny;
+  const uint32_t tt = tiles[first + blockIdx.x];
+  const uint32_t w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const uint32_t X = (tt & 0xffffu) * 16 + (w & 1) * 8 + (l & 7);
+  const uint32_t Y = (tt >> 16) * 16 + (w >> 1) * 8 + (l >> 3);
+  if (X >= nx || Y >= ny) return;
+  const uint32_t e0 = blockIdx.z * NE;
+  const double fill = rot_mask ? 0.0 : __longlong_as_double(0x7ff8000000000000ll);
+#pragma unroll
+  for (int e = 0; e < NE; e++)
+    if (e0 + e < ne) avg[(uint64_t)(e0 + e) * P + (uint64_t)Y * nx + X] = fill;
+}
+
 // Quarter-turn orbits (square N x N planes whose angle list holds Q quarter turns
 // of A4 steps: angle a + q A4 is angle a plus 90q degrees).  The image rotation
 // turns about (N/2, N/2), so with s(X, Y) = (N - Y, X) (a 90-degree turn of the
@@ -554,6 +574,19 @@ Inv6 mk_inv(const double inv[6]) {
 
 uint32_t rotation_tile_count(uint32_t nx, uint32_t ny) { return ceil_div(nx, 16) * ceil_div(ny, 16); }
 
+// Tiles of rotation_tile_order whose centre lies within r pixels of the plane's
+// centre: the first ones of that order (same key).
+uint32_t rotation_tiles_within(uint32_t nx, uint32_t ny, double r) {
+  const uint32_t tx = ceil_div(nx, 16), ty = ceil_div(ny, 16);
+  uint32_t c = 0;
+  for (uint32_t y = 0; y < ty; y++)
+    for (uint32_t x = 0; x < tx; x++) {
+      const double dx = (x * 16.0 + 8.0) - nx / 2.0, dy = (y * 16.0 + 8.0) - ny / 2.0;
+      c += dx * dx + dy * dy <= r * r ? 1u : 0u;
+    }
+  return c;
+}
+
 void rotation_tile_order(uint32_t nx, uint32_t ny, uint32_t *out) {
   const uint32_t tx = ceil_div(nx, 16), ty = ceil_div(ny, 16);
   std::vector<std::pair<double, uint32_t>> r;
@@ -689,16 +722,30 @@ cyr_status launch_rotate_quad(const T *quad, const double *inv_tab, const double
     return CYR_OK;
   }
   const uint32_t ntiles = rotation_tile_count(nx, ny);
+  // tiles (in order of distance from the centre (Nx/2, Ny/2), about which the
+  // image turns) whose pixels all lie more than rfin + 2 pixels out are filled:
+  // a 16 x 16 tile's pixels are within 8 sqrt(2) of its centre
+  const uint32_t n_act = rfin >= 0 ? std::min(ntiles, rotation_tiles_within(nx, ny, rfin + 14.0)) : ntiles;
   // Energies per pass: 4 share each angle's taps; once a plane of quadratic forms
   // exceeds 16 MB (2048^2: 64 MB), one energy per pass keeps the gathers' working
   // set in cache (measured 19% faster there, equal at 512^2).
-  if ((uint64_t)nx * ny * kQuadRecord * sizeof(T) > (16ull << 20))
-    hipLaunchKernelGGL((rotate_quad_kernel<T, 1>), dim3(xcd_grid(ntiles), 1, ne), dim3(256), 0, st, quad, inv_tab,
-                       coef, nA, nx, ny, rot_mask, avg, ne, tiles, ntiles);
+  const bool one = (uint64_t)nx * ny * kQuadRecord * sizeof(T) > (16ull << 20);
+  if (one)
+    hipLaunchKernelGGL((rotate_quad_kernel<T, 1>), dim3(xcd_grid(n_act), 1, ne), dim3(256), 0, st, quad, inv_tab,
+                       coef, nA, nx, ny, rot_mask, avg, ne, tiles, n_act);
   else
-    hipLaunchKernelGGL((rotate_quad_kernel<T, 4>), dim3(xcd_grid(ntiles), 1, ceil_div(ne, 4)), dim3(256), 0, st, quad,
-                       inv_tab, coef, nA, nx, ny, rot_mask, avg, ne, tiles, ntiles);
+    hipLaunchKernelGGL((rotate_quad_kernel<T, 4>), dim3(xcd_grid(n_act), 1, ceil_div(ne, 4)), dim3(256), 0, st, quad,
+                       inv_tab, coef, nA, nx, ny, rot_mask, avg, ne, tiles, n_act);
   CYR_LAUNCHED();
+  if (n_act < ntiles) {
+    if (one)
+      hipLaunchKernelGGL((quad_fill_kernel<1>), dim3(ntiles - n_act, 1, ne), dim3(256), 0, st, avg, nx, ny, ne, rot_mask,
+                         tiles, n_act);
+    else
+      hipLaunchKernelGGL((quad_fill_kernel<4>), dim3(ntiles - n_act, 1, ceil_div(ne, 4)), dim3(256), 0, st, avg, nx, ny,
+                         ne, rot_mask, tiles, n_act);
+    CYR_LAUNCHED();
+  }
   return CYR_OK;
 }
 
