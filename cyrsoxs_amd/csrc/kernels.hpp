@@ -230,7 +230,7 @@ void rotation_tile_order(uint32_t nx, uint32_t ny, uint32_t *out);
 // (tiles: rotation_tile_order on the device).
 template <typename T>
 cyr_status launch_rotate_all(const T *proj, const double *inv_tab, uint32_t nA, uint32_t nx, uint32_t ny,
-                             int rot_mask, double *avg, const uint32_t *tiles, hipStream_t st);
+                             int rot_mask, double *avg, const uint32_t *tiles, double rfin, hipStream_t st);
 // Quarter-turn orbits of a square N x N plane (N even): orbit representatives
 // in 16 x 16 tiles (ty << 16 | tx) ordered by distance from the rotation centre.
 uint32_t rotation_orbit_tile_count(uint32_t n);

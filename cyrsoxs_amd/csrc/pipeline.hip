@@ -455,7 +455,7 @@ cyr_status run_ebasis(const cyr_sim_desc *d, const SimGeometry &g, const void *d
     kt_end("ebasis_angles", s, part_bytes + (double)g.nA * P * sizeof(T));
     kt_begin(s);
     CYR_TRY(launch_rotate_all<T>(proj, su.rotinv + (size_t)6 * k * g.nA, g.nA, g.nx, g.ny, d->rot_mask, avg,
-                                 su.tiles, s));
+                                 su.tiles, -1.0, s));
     kt_end("rotate_all", s, (double)(g.nA + 1) * P * sizeof(T));
     CYR_TRY(launch_detector_warp<T>(avg, out, g.nx, g.ny, &g.det_inv[(size_t)k * 6], d->rot_mask, s));
   }
@@ -737,7 +737,8 @@ cyr_status run_mbasis(const cyr_sim_desc *d, const SimGeometry &g, const void *d
           CYR_TRY(launch_mbasis_angles<T>(aa, ng, s));
           kt_end("mbasis_angles", s, 2.0 * ng * nvol * (double)P * cxb + (double)g.nA * P * sizeof(T));
           kt_begin(s);
-          CYR_TRY(launch_rotate_all<T>(proj, inv, g.nA, g.nx, g.ny, d->rot_mask, avg, su.tiles, s));
+          CYR_TRY(launch_rotate_all<T>(proj, inv, g.nA, g.nx, g.ny, d->rot_mask, avg, su.tiles,
+                                       0.5 * std::sqrt((double)d4), s));
           kt_end("rotate_all", s, (double)(g.nA + 1) * P * sizeof(T));
           T *out = d_out + ((uint64_t)e * g.nK + k) * P;
           CYR_TRY(launch_detector_warp<T>(avg, out, g.nx, g.ny, &g.det_inv[(size_t)k * 6], d->rot_mask, s));

@@ -655,12 +655,20 @@ cyr_status launch_average(const double *acc, const uint32_t *count, double *avg,
 
 template <typename T>
 cyr_status launch_rotate_all(const T *proj, const double *inv_tab, uint32_t nA, uint32_t nx, uint32_t ny, int rot_mask,
-                             double *avg, const uint32_t *tiles, hipStream_t st) {
+                             double *avg, const uint32_t *tiles, double rfin, hipStream_t st) {
   const uint32_t ntiles = rotation_tile_count(nx, ny);
   if (!tiles) return fail(CYR_E_INVALID, "rotation: tile order missing");
-  hipLaunchKernelGGL((rotate_all_kernel<T>), dim3(xcd_grid(ntiles)), dim3(256), 0, st, proj, inv_tab, nA, nx, ny,
-                     rot_mask, avg, tiles, ntiles);
+  // the finite-radius cut of launch_rotate_quad (the projections are NaN where the
+  // forms are)
+  const uint32_t n_act = rfin >= 0 ? std::min(ntiles, rotation_tiles_within(nx, ny, rfin + 14.0)) : ntiles;
+  hipLaunchKernelGGL((rotate_all_kernel<T>), dim3(xcd_grid(n_act)), dim3(256), 0, st, proj, inv_tab, nA, nx, ny,
+                     rot_mask, avg, tiles, n_act);
   CYR_LAUNCHED();
+  if (n_act < ntiles) {
+    hipLaunchKernelGGL((quad_fill_kernel<1>), dim3(ntiles - n_act), dim3(256), 0, st, avg, nx, ny, 1u, rot_mask, tiles,
+                       n_act);
+    CYR_LAUNCHED();
+  }
   return CYR_OK;
 }
 
@@ -789,7 +797,7 @@ cyr_status launch_warp(const T *src, T *dst, uint32_t nx, uint32_t ny, const dou
   template cyr_status launch_rotate_accumulate<T>(const T *, double *, uint32_t *, uint32_t, uint32_t,             \
                                                   const double *, int, hipStream_t);                               \
   template cyr_status launch_rotate_all<T>(const T *, const double *, uint32_t, uint32_t, uint32_t, int, double *, \
-                                           const uint32_t *, hipStream_t);                                         \
+                                           const uint32_t *, double, hipStream_t);                                 \
   template cyr_status launch_rotate_quad<T>(const T *, const double *, const double *, const T *, uint32_t,      \
                                             uint32_t, uint32_t, int, double *, uint32_t, const uint32_t *,        \
                                             uint32_t, const uint32_t *, bool, double, hipStream_t);                \

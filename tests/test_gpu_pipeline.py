@@ -87,6 +87,10 @@ CASES = {
     "span_extract": dict(dims=(64, 64, 32), M=2, morph=0, kw=dict(interpolation=1), phys=1.4),
     "span_material_frame": dict(dims=(64, 48, 64), M=2, morph=0, kw=dict(interpolation=1, reference_frame=0),
                                 phys=1.5),
+    # material frame with the rotation mask on a 2D film at 1 nm (most of the detector outside
+    # the Ewald sphere: the per-angle projections' rotation fills the tiles beyond it)
+    "span_material_rotmask": dict(dims=(48, 48, 1), M=2, morph=1,
+                                  kw=dict(interpolation=1, rot_mask=1, reference_frame=0)),
     # odd Nx (run-time X plan; the bin -> shifted map of odd lengths; a ragged last tile)
     "span_odd_nx": dict(dims=(45, 64, 64), M=2, morph=1, kw=dict(interpolation=1, windowing=1), phys=1.3),
 }
